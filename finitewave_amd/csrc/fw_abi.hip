@@ -1,0 +1,399 @@
+// fw_abi.hip -- the extern "C" surface declared in include/finitewave_amd.h.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "fw_internal.hpp"
+
+namespace fw {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+#define FW_DECL_LAUNCHER(p, m)                                                 \
+  int fw_step_launch_p##p##_m##m(const fw_plan*, const void*, void*,           \
+                                 const StatePtrs&, const TrackArgs&, hipStream_t);
+FW_DECL_LAUNCHER(0, 0) FW_DECL_LAUNCHER(0, 1) FW_DECL_LAUNCHER(0, 2) FW_DECL_LAUNCHER(0, 3)
+FW_DECL_LAUNCHER(0, 4) FW_DECL_LAUNCHER(0, 5) FW_DECL_LAUNCHER(0, 6) FW_DECL_LAUNCHER(0, 7)
+FW_DECL_LAUNCHER(1, 0) FW_DECL_LAUNCHER(1, 1) FW_DECL_LAUNCHER(1, 2) FW_DECL_LAUNCHER(1, 3)
+FW_DECL_LAUNCHER(1, 4) FW_DECL_LAUNCHER(1, 5) FW_DECL_LAUNCHER(1, 6) FW_DECL_LAUNCHER(1, 7)
+
+StepLauncher step_launcher(int precision, int model) {
+  static const StepLauncher table[2][8] = {
+      {fw_step_launch_p0_m0, fw_step_launch_p0_m1, fw_step_launch_p0_m2, fw_step_launch_p0_m3,
+       fw_step_launch_p0_m4, fw_step_launch_p0_m5, fw_step_launch_p0_m6, fw_step_launch_p0_m7},
+      {fw_step_launch_p1_m0, fw_step_launch_p1_m1, fw_step_launch_p1_m2, fw_step_launch_p1_m3,
+       fw_step_launch_p1_m4, fw_step_launch_p1_m5, fw_step_launch_p1_m6, fw_step_launch_p1_m7}};
+  if (precision < 0 || precision > 1 || model < 0 || model > 7) return nullptr;
+  return table[precision][model];
+}
+
+__global__ void k_set_word(int32_t* p, int32_t v) { *p = v; }
+
+}  // namespace fw
+
+using namespace fw;
+
+static hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+static StatePtrs make_state(void* const* state, int n) {
+  StatePtrs st;
+  for (int k = 0; k < FW_MAX_STATE; ++k) st.p[k] = k < n ? state[k] : nullptr;
+  return st;
+}
+
+extern "C" {
+
+int fw_abi_version(void) { return FW_ABI_VERSION; }
+
+const char* fw_last_error(void) { return g_err; }
+
+int fw_model_info(int32_t model, int32_t* n_state, int32_t* n_params) {
+  if (model < 0 || model >= FW_N_MODELS) {
+    set_error("unknown model %d", model);
+    return FW_E_INVALID;
+  }
+  if (n_state) *n_state = model_n_state(model);
+  if (n_params) *n_params = model_n_params(model);
+  return FW_OK;
+}
+
+int fw_state_is_f64(int32_t model, int32_t precision, int32_t k) {
+  if (model < 0 || model >= FW_N_MODELS || k < 0 || k >= model_n_state(model)) return FW_E_INVALID;
+  return state_is_f64(model, precision, k) ? 1 : 0;
+}
+
+int fw_compute_weights(int32_t ndim, int32_t kind, double* w, const int8_t* m,
+                       const double* const* d, const int64_t* shape, void* stream) {
+  if (!w || !m || !d || !shape || (ndim != 2 && ndim != 3)) {
+    set_error("fw_compute_weights: bad arguments");
+    return FW_E_INVALID;
+  }
+  return launch_weights(ndim, kind, w, m, d, shape, as_stream(stream));
+}
+
+int fw_diffusion(int32_t ndim, int32_t K, double* u_new, const double* u, const double* w,
+                 const int64_t* indexes, int64_t n_indexes, const int64_t* shape,
+                 void* stream) {
+  if (!u_new || !u || !w || (!indexes && n_indexes) || !shape || n_indexes < 0) {
+    set_error("fw_diffusion: bad arguments");
+    return FW_E_INVALID;
+  }
+  return launch_diffusion(ndim, K, u_new, u, w, indexes, n_indexes, shape, as_stream(stream));
+}
+
+int fw_ionic(int32_t model, int32_t precision, void* u_new, const void* u, void* const* state,
+             int32_t n_state, const int64_t* indexes, int64_t n_indexes, double dt,
+             const double* params, int32_t n_params, void* stream) {
+  if (model < 0 || model >= FW_N_MODELS) {
+    set_error("fw_ionic: unknown model %d", model);
+    return FW_E_INVALID;
+  }
+  if (n_state != model_n_state(model) || n_params != model_n_params(model)) {
+    set_error("fw_ionic: model %d takes %d state arrays and %d parameters (got %d, %d)", model,
+              model_n_state(model), model_n_params(model), n_state, n_params);
+    return FW_E_INVALID;
+  }
+  if (precision != FW_PREC_FP64 && precision != FW_PREC_FP32) {
+    set_error("fw_ionic: bad precision %d", precision);
+    return FW_E_INVALID;
+  }
+  if (!u_new || !u || !state || (!indexes && n_indexes) || !params || n_indexes < 0) {
+    set_error("fw_ionic: null argument");
+    return FW_E_INVALID;
+  }
+  for (int k = 0; k < n_state; ++k)
+    if (!state[k]) {
+      set_error("fw_ionic: state %d is null", k);
+      return FW_E_INVALID;
+    }
+  Params P;
+  for (int q = 0; q < FW_MAX_PARAMS; ++q) P.v[q] = q < n_params ? params[q] : 0.0;
+  return launch_ionic(model, precision, u_new, u, make_state(state, n_state), indexes,
+                      n_indexes, dt, P, as_stream(stream));
+}
+
+int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
+  if (!desc || !out) {
+    set_error("fw_plan_create: null argument");
+    return FW_E_INVALID;
+  }
+  *out = nullptr;
+  const fw_plan_desc& d = *desc;
+  if (d.ndim != 2 && d.ndim != 3) {
+    set_error("fw_plan_create: ndim must be 2 or 3 (got %d)", d.ndim);
+    return FW_E_INVALID;
+  }
+  if (d.model < 0 || d.model >= FW_N_MODELS) {
+    set_error("fw_plan_create: unknown model %d", d.model);
+    return FW_E_INVALID;
+  }
+  if (d.precision != FW_PREC_FP64 && d.precision != FW_PREC_FP32) {
+    set_error("fw_plan_create: bad precision %d", d.precision);
+    return FW_E_INVALID;
+  }
+  if (d.stencil < FW_STENCIL_ISO || d.stencil > FW_STENCIL_WEIGHTS) {
+    set_error("fw_plan_create: bad stencil %d", d.stencil);
+    return FW_E_INVALID;
+  }
+  const int need_fields = d.stencil == FW_STENCIL_WEIGHTS ? 0
+                          : d.stencil == FW_STENCIL_ISO  ? d.ndim
+                          : (d.ndim == 2 ? 4 : 9);
+  if (d.n_fields < need_fields || d.n_fields > FW_MAX_FIELDS) {
+    set_error("fw_plan_create: stencil needs %d half-step fields (got %d)", need_fields,
+              d.n_fields);
+    return FW_E_INVALID;
+  }
+  for (int q = 0; q < need_fields; ++q) {
+    const fw_field& f = d.fields[q];
+    if (f.mode < FW_FIELD_UNIFORM || f.mode > FW_FIELD_FULL ||
+        (f.mode != FW_FIELD_UNIFORM && !f.data) ||
+        (f.mode == FW_FIELD_AXIS && (f.axis < 0 || f.axis >= d.ndim))) {
+      set_error("fw_plan_create: field %d is malformed", q);
+      return FW_E_INVALID;
+    }
+  }
+  if (d.stencil == FW_STENCIL_WEIGHTS && !d.weights) {
+    set_error("fw_plan_create: FW_STENCIL_WEIGHTS needs a weights array");
+    return FW_E_INVALID;
+  }
+  if (d.n_params != model_n_params(d.model)) {
+    set_error("fw_plan_create: model %d takes %d parameters (got %d)", d.model,
+              model_n_params(d.model), d.n_params);
+    return FW_E_INVALID;
+  }
+  if (!d.code) {
+    set_error("fw_plan_create: code array is null");
+    return FW_E_INVALID;
+  }
+  if (!(d.dt > 0) || !(d.dr2 > 0)) {
+    set_error("fw_plan_create: dt and dr2 must be positive");
+    return FW_E_INVALID;
+  }
+  int64_t N = 1;
+  for (int a = 0; a < d.ndim; ++a) {
+    if (d.shape[a] < 3) {
+      set_error("fw_plan_create: every axis needs at least 3 nodes (axis %d has %lld)", a,
+                (long long)d.shape[a]);
+      return FW_E_INVALID;
+    }
+    N *= d.shape[a];
+  }
+  if (N > ((int64_t)1 << 40)) {
+    set_error("fw_plan_create: grid too large");
+    return FW_E_INVALID;
+  }
+  fw_plan* p = new (std::nothrow) fw_plan();
+  if (!p) {
+    set_error("fw_plan_create: out of host memory");
+    return FW_E_NOMEM;
+  }
+  p->desc = d;
+  if (d.ndim == 2) p->desc.shape[2] = 1;
+  p->N = N;
+  p->nblocks = (uint32_t)((N + kBlock - 1) / kBlock);
+  p->per_xcd = (p->nblocks + kXcds - 1) / kXcds;
+  p->grid = p->per_xcd * kXcds;
+  p->K = d.ndim == 2 ? 9 : 19;
+  p->graph = nullptr;
+  p->graph_steps = 0;
+  hipError_t e = hipMalloc(&p->step_words, 2 * sizeof(int32_t));
+  if (e != hipSuccess) {
+    set_error("fw_plan_create: hipMalloc failed: %s", hipGetErrorString(e));
+    delete p;
+    return FW_E_NOMEM;
+  }
+  *out = p;
+  return FW_OK;
+}
+
+// the K of a FW_STENCIL_WEIGHTS plan: 5/9 (2D) or 7/19 (3D) weights per node
+int fw_plan_set_weights_k(fw_plan* plan, int32_t K) {
+  if (!plan) return FW_E_INVALID;
+  const int nd = plan->desc.ndim;
+  if (!((nd == 2 && (K == 5 || K == 9)) || (nd == 3 && (K == 7 || K == 19)))) {
+    set_error("fw_plan_set_weights_k: K=%d does not fit ndim=%d", K, nd);
+    return FW_E_INVALID;
+  }
+  plan->K = K;
+  return FW_OK;
+}
+
+int fw_plan_destroy(fw_plan* plan) {
+  if (!plan) return FW_OK;
+  if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
+  if (plan->step_words) (void)hipFree(plan->step_words);
+  delete plan;
+  return FW_OK;
+}
+
+static int check_run_args(fw_plan* plan, void* u, void* u_new, void* const* state,
+                          int32_t n_steps, const fw_track_desc* track) {
+  if (!plan || !u || !u_new || !state || n_steps < 0) {
+    set_error("fw_run: bad arguments");
+    return FW_E_INVALID;
+  }
+  const int ns = model_n_state(plan->desc.model);
+  for (int k = 0; k < ns; ++k)
+    if (!state[k]) {
+      set_error("fw_run: state %d is null", k);
+      return FW_E_INVALID;
+    }
+  if (track && track->n_probe > 0 &&
+      (!track->probe_idx || !track->probe_var || !track->probe_buf)) {
+    set_error("fw_run: probes need idx, var and buf");
+    return FW_E_INVALID;
+  }
+  return FW_OK;
+}
+
+int fw_run(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
+           int32_t n_steps, const fw_track_desc* track, void* stream) {
+  int rc = check_run_args(plan, u, u_new, state, n_steps, track);
+  if (rc) return rc;
+  StepLauncher L = step_launcher(plan->desc.precision, plan->desc.model);
+  const StatePtrs st = make_state(state, model_n_state(plan->desc.model));
+  void* a = u;
+  void* b = u_new;
+  for (int32_t s = 0; s < n_steps; ++s) {
+    TrackArgs tr{};
+    tr.step = (int32_t)(step0 + s);
+    if (track) {
+      tr.act = track->act;
+      tr.act_thr = track->act_thr;
+      tr.act_on = track->act && (!track->act_flags || track->act_flags[s]);
+      tr.n_probe = track->n_probe;
+      tr.probe_idx = track->probe_idx;
+      tr.probe_var = track->probe_var;
+      tr.probe_buf = track->probe_buf;
+      tr.probe_row = track->probe_rows ? track->probe_rows[s] : -1;
+    } else {
+      tr.probe_row = -1;
+    }
+    rc = L(plan, a, b, st, tr, as_stream(stream));
+    if (rc) return rc;
+    void* t = a;
+    a = b;
+    b = t;
+  }
+  return FW_OK;
+}
+
+int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
+                 int32_t n_steps, const fw_track_desc* track, void* stream) {
+  int rc = check_run_args(plan, u, u_new, state, n_steps, track);
+  if (rc) return rc;
+  if (track && track->n_probe > 0) {
+    set_error("fw_run_graph: probes need per-step rows; use fw_run");
+    return FW_E_UNSUPPORTED;
+  }
+  int32_t act_on = 0;
+  int32_t* act = nullptr;
+  if (track && track->act) {
+    act = track->act;
+    act_on = track->act_flags ? track->act_flags[0] : 1;
+    if (track->act_flags)
+      for (int32_t s = 1; s < n_steps; ++s)
+        if (!!track->act_flags[s] != !!act_on) {
+          set_error("fw_run_graph: the tracker gate must not change within the run");
+          return FW_E_UNSUPPORTED;
+        }
+  }
+  const int ns = model_n_state(plan->desc.model);
+  const int32_t pairs = n_steps / 2;
+  hipStream_t s = as_stream(stream);
+  // cached graph still valid for these buffers?
+  bool reuse = plan->graph && plan->graph_u == u && plan->graph_un == u_new &&
+               plan->graph_act == act && plan->graph_act_on == act_on;
+  for (int k = 0; reuse && k < ns; ++k) reuse = plan->graph_state[k] == state[k];
+  if (!reuse && pairs > 0) {
+    if (plan->graph) {
+      (void)hipGraphExecDestroy(plan->graph);
+      plan->graph = nullptr;
+    }
+    StepLauncher L = step_launcher(plan->desc.precision, plan->desc.model);
+    const StatePtrs st = make_state(state, ns);
+    hipStream_t cap;
+    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+      set_error("fw_run_graph: stream creation failed");
+      return FW_E_HIP;
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      TrackArgs tr{};
+      tr.act = act;
+      tr.act_thr = track ? track->act_thr : 0.0;
+      tr.act_on = act_on;
+      tr.probe_row = -1;
+      tr.step_in = plan->step_words;
+      tr.step_out = plan->step_words + 1;
+      rc = L(plan, u, u_new, st, tr, cap);
+      tr.step_in = plan->step_words + 1;
+      tr.step_out = plan->step_words;
+      if (!rc) rc = L(plan, u_new, u, st, tr, cap);
+      e = hipStreamEndCapture(cap, &g);
+    }
+    hipGraphExec_t ex = nullptr;
+    if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipStreamDestroy(cap);
+    if (e != hipSuccess || rc || !ex) {
+      set_error("fw_run_graph: capture failed (%s)", hipGetErrorString(e));
+      return rc ? rc : FW_E_HIP;
+    }
+    plan->graph = ex;
+    plan->graph_u = u;
+    plan->graph_un = u_new;
+    plan->graph_act = act;
+    plan->graph_act_on = act_on;
+    for (int k = 0; k < FW_MAX_STATE; ++k) plan->graph_state[k] = k < ns ? state[k] : nullptr;
+  }
+  if (pairs > 0) {
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, plan->step_words, (int32_t)step0);
+    for (int32_t q = 0; q < pairs; ++q) {
+      hipError_t e = hipGraphLaunch(plan->graph, s);
+      if (e != hipSuccess) {
+        set_error("fw_run_graph: launch failed: %s", hipGetErrorString(e));
+        return FW_E_HIP;
+      }
+    }
+  }
+  if (n_steps % 2) {
+    // the last odd step runs as a plain launch on the original orientation
+    fw_track_desc one{};
+    uint8_t flag = (uint8_t)act_on;
+    if (track) {
+      one = *track;
+      one.act_flags = &flag;
+    }
+    return fw_run(plan, u, u_new, state, step0 + 2 * pairs, 1, track ? &one : nullptr, stream);
+  }
+  return FW_OK;
+}
+
+int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim, void* stream) {
+  if (!plan || !u || !stim || (stim->kind != 0 && stim->kind != 1)) {
+    set_error("fw_stim_apply: bad arguments");
+    return FW_E_INVALID;
+  }
+  return launch_stim(plan->desc.precision, u, plan->desc.code, stim, plan->desc.shape,
+                     as_stream(stream));
+}
+
+int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr, int32_t step,
+                       void* stream) {
+  if (!plan || !act || !u) {
+    set_error("fw_act_time_update: bad arguments");
+    return FW_E_INVALID;
+  }
+  return launch_act(plan->desc.precision, act, u, plan->N, thr, step, as_stream(stream));
+}
+
+}  // extern "C"

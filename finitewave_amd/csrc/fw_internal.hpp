@@ -1,0 +1,67 @@
+// fw_internal.hpp -- declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/finitewave_amd.h"
+#include "fw_models.hpp"
+
+struct fw_plan {
+  fw_plan_desc desc;
+  int64_t N;
+  uint32_t nblocks;  // logical blocks of 256 nodes
+  uint32_t per_xcd;  // logical blocks per XCD
+  uint32_t grid;     // launched blocks = per_xcd * 8
+  int32_t K;         // weights per node for FW_STENCIL_WEIGHTS
+  int32_t* step_words;  // device int32[2] for graph replay
+  hipGraphExec_t graph;
+  int32_t graph_steps;
+  const void* graph_u;
+  const void* graph_un;
+  void* graph_state[FW_MAX_STATE];
+  int32_t graph_act_on;
+  int32_t* graph_act;
+};
+
+namespace fw {
+
+struct TrackArgs {
+  int32_t* act;
+  double act_thr;
+  int32_t act_on;
+  int32_t step;
+  const int32_t* step_in;
+  int32_t* step_out;
+  int32_t n_probe;
+  const int64_t* probe_idx;
+  const int32_t* probe_var;
+  double* probe_buf;
+  int32_t probe_row;
+};
+
+typedef int (*StepLauncher)(const fw_plan* plan, const void* u, void* u_new,
+                            const StatePtrs& st, const TrackArgs& tr,
+                            hipStream_t stream);
+
+// one launcher per (precision, model), defined in fw_step_inst.hip builds
+StepLauncher step_launcher(int precision, int model);
+
+// literal sweeps (fw_literal.hip)
+int launch_weights(int ndim, int kind, double* w, const int8_t* m,
+                   const double* const* d_dev, const int64_t* shape,
+                   hipStream_t s);
+int launch_diffusion(int ndim, int K, double* u_new, const double* u,
+                     const double* w, const int64_t* idx, int64_t n,
+                     const int64_t* shape, hipStream_t s);
+int launch_ionic(int model, int precision, void* u_new, const void* u,
+                 const StatePtrs& st, const int64_t* idx, int64_t n,
+                 double dt, const Params& P, hipStream_t s);
+int launch_stim(int precision, void* u, const uint8_t* code,
+                const fw_stim_desc* stim, const int64_t* shape, hipStream_t s);
+int launch_act(int precision, int32_t* act, const void* u, int64_t n,
+               double thr, int32_t step, hipStream_t s);
+
+void set_error(const char* fmt, ...);
+
+}  // namespace fw

@@ -1,0 +1,732 @@
+// fw_kernels.hpp -- gfx950 kernels of the stepping engine.
+//
+// One fused kernel per time step: each work-item owns one grid node, rebuilds
+// its stencil weights from the uint8 node codes (bit 0 = conducting) and the
+// half-step diffusion fields, forms the diffusion sum, applies the ionic
+// update and writes u_new -- one pass over HBM per step.  Non-updated nodes
+// still run the activation-time test (the reference tracker scans every
+// node) and point probes.
+//
+// Two stencil arithmetics:
+//  * FP64: the reference's weights exactly -- the same sequence of += / -=
+//    into zeroed weights (asymmetric_stencil_2d.py:291-433 and friends), the
+//    numpy epilogue ((w*D)*dt)/dr**2 with +1 on the centre, and the
+//    left-to-right weighted sum of the diffusion kernel.
+//  * FP32: the same operator in flux form, u + s*sum(faces), s = D*dt/dr**2,
+//    which avoids the 1 - O(1) cancellation of the centre weight in float.
+//
+// Grid: one flat 1-D range of nodes, 256 work-items per block, block ids
+// remapped so each XCD walks a contiguous slab of rows (neighbour rows then
+// hit that XCD's own L2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fw_models.hpp"
+
+namespace fw {
+
+struct Field {
+  int32_t mode;  // 0 uniform, 1 per-axis table, 2 full
+  int32_t axis;
+  double value;
+  const double* data;
+};
+
+__device__ __forceinline__ double fval(const Field& f, int64_t i, int64_t j,
+                                       int64_t k, int64_t c) {
+  if (f.mode == 0) return f.value;
+  if (f.mode == 1) return f.data[f.axis == 0 ? i : (f.axis == 1 ? j : k)];
+  return f.data[c];
+}
+
+template <typename R>
+struct KArgs {
+  const R* __restrict__ u;
+  R* __restrict__ u_new;
+  StatePtrs st;
+  const uint8_t* __restrict__ code;
+  int64_t ni, nj, nk, N;
+  uint32_t nblocks;  // logical blocks covering N
+  uint32_t per_xcd;  // logical blocks per XCD
+  Field f[9];
+  const double* __restrict__ w;  // precomputed weights (stencil 2)
+  double D, dtd, dr2;
+  R dt, s;
+  Params P;
+  int32_t* act;
+  R act_thr;
+  int32_t act_on;
+  int32_t step;
+  const int32_t* step_in;  // graph replay: step read from / written to device
+  int32_t* step_out;
+  int32_t n_probe;
+  const int64_t* probe_idx;
+  const int32_t* probe_var;
+  double* probe_buf;
+  int32_t probe_row;
+};
+
+// logical block id with each XCD owning a contiguous run of blocks
+__device__ __forceinline__ int64_t logical_block(uint32_t per_xcd) {
+  const uint32_t b = blockIdx.x;
+  return (int64_t)(b % kXcds) * per_xcd + (b / kXcds);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 exact weights
+// ---------------------------------------------------------------------------
+
+// isotropic_stencil_2d.py:135-158: d * m0 * (m0 + (m1 == 0))
+__device__ __forceinline__ double iso_comp(double d, int m0, int m1) {
+  return d * (double)m0 * (double)(m0 + (m1 == 0));
+}
+
+// asymmetric_stencil_2d.py:204-259 for 0/1 mesh values: zeros unless the two
+// face nodes conduct and both 4-node sides hold at least 3 conducting nodes.
+// Zeros are returned as +0.0; adding or subtracting a signed zero leaves
+// the accumulated weights unchanged, so this matches the reference bits.
+__device__ __forceinline__ void minor_exact(double d, int m0, int m1, int m2,
+                                            int m3, int m4, int m5,
+                                            double q[6]) {
+  const int mh = m2 + m3 + m4 + m5;
+  const int ml = m0 + m1 + m2 + m3;
+  if (m2 == 0 || m3 == 0 || mh < 3 || ml < 3) {
+    q[0] = q[1] = q[2] = q[3] = q[4] = q[5] = 0.0;
+    return;
+  }
+  const double dmh = (double)mh, dml = (double)ml;
+  q[0] = m0 ? (-d) / dml : 0.0;
+  q[1] = m1 ? (-d) / dml : 0.0;
+  const double c = 1.0 / dmh - 1.0 / dml;  // m2 = m3 = 1 here
+  q[2] = d * c;
+  q[3] = d * c;
+  q[4] = m4 ? d / dmh : 0.0;
+  q[5] = m5 ? d / dmh : 0.0;
+}
+
+// numpy epilogue: w = w * D * dt / dr**2 ; w[..., centre] += 1
+template <int K>
+__device__ __forceinline__ void scale_exact(double W[K], int centre, double D,
+                                            double dt, double dr2) {
+#pragma unroll
+  for (int q = 0; q < K; ++q) W[q] = W[q] * D * dt / dr2;
+  W[centre] += 1.0;
+}
+
+// 3x3(x3) neighbourhood of conduct bits, index (di+1)*9 + (dj+1)*3 + (dk+1)
+struct Nbr {
+  int m[27];
+  __device__ __forceinline__ int operator()(int di, int dj, int dk) const {
+    return m[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)];
+  }
+};
+
+// asymmetric_stencil_2d.py:291-433 (2D: dk = 0 everywhere)
+__device__ __forceinline__ void wts_aniso2d(double W[9], const Nbr& M,
+                                            double dxx0, double dxx1,
+                                            double dxy0, double dxy1,
+                                            double dyx0, double dyx1,
+                                            double dyy0, double dyy1) {
+#pragma unroll
+  for (int q = 0; q < 9; ++q) W[q] = 0.0;
+  double q[6], qm;
+  qm = dxx0 * (double)M(-1, 0, 0);
+  W[1] += qm; W[4] -= qm;
+  minor_exact(dxy0, M(-1, -1, 0), M(0, -1, 0), M(-1, 0, 0), M(0, 0, 0),
+              M(-1, 1, 0), M(0, 1, 0), q);
+  W[0] -= q[0]; W[3] -= q[1]; W[1] -= q[2]; W[4] -= q[3]; W[2] -= q[4]; W[5] -= q[5];
+  qm = dyy0 * (double)M(0, -1, 0);
+  W[3] += qm; W[4] -= qm;
+  minor_exact(dyx0, M(-1, -1, 0), M(-1, 0, 0), M(0, -1, 0), M(0, 0, 0),
+              M(1, -1, 0), M(1, 0, 0), q);
+  W[0] -= q[0]; W[1] -= q[1]; W[3] -= q[2]; W[4] -= q[3]; W[6] -= q[4]; W[7] -= q[5];
+  qm = dyy1 * (double)M(0, 1, 0);
+  W[5] += qm; W[4] -= qm;
+  minor_exact(dyx1, M(-1, 1, 0), M(-1, 0, 0), M(0, 1, 0), M(0, 0, 0),
+              M(1, 1, 0), M(1, 0, 0), q);
+  W[2] += q[0]; W[1] += q[1]; W[5] += q[2]; W[4] += q[3]; W[8] += q[4]; W[7] += q[5];
+  qm = dxx1 * (double)M(1, 0, 0);
+  W[7] += qm; W[4] -= qm;
+  minor_exact(dxy1, M(1, -1, 0), M(0, -1, 0), M(1, 0, 0), M(0, 0, 0),
+              M(1, 1, 0), M(0, 1, 0), q);
+  W[6] += q[0]; W[3] += q[1]; W[7] += q[2]; W[4] += q[3]; W[8] += q[4]; W[5] += q[5];
+}
+
+// asymmetric_stencil_3d.py:163-458; d[face][comp]: faces x0 x1 y0 y1 z0 z1,
+// comps: x faces (xx, xy, xz), y faces (yy, yx, yz), z faces (zz, zx, zy)
+__device__ __forceinline__ void wts_aniso3d(double W[19], const Nbr& M,
+                                            const double d[6][3]) {
+#pragma unroll
+  for (int q = 0; q < 19; ++q) W[q] = 0.0;
+  double q[6], qm;
+  // q (i-1/2, j, k)
+  qm = d[0][0] * (double)M(-1, 0, 0);
+  W[1] += qm; W[4] -= qm;
+  minor_exact(d[0][1], M(-1, -1, 0), M(0, -1, 0), M(-1, 0, 0), M(0, 0, 0),
+              M(-1, 1, 0), M(0, 1, 0), q);
+  W[0] -= q[0]; W[3] -= q[1]; W[1] -= q[2]; W[4] -= q[3]; W[2] -= q[4]; W[5] -= q[5];
+  minor_exact(d[0][2], M(-1, 0, -1), M(0, 0, -1), M(-1, 0, 0), M(0, 0, 0),
+              M(-1, 0, 1), M(0, 0, 1), q);
+  W[15] -= q[0]; W[11] -= q[1]; W[1] -= q[2]; W[4] -= q[3]; W[17] -= q[4]; W[12] -= q[5];
+  // q (i+1/2, j, k)
+  qm = d[1][0] * (double)M(1, 0, 0);
+  W[7] += qm; W[4] -= qm;
+  minor_exact(d[1][1], M(1, -1, 0), M(0, -1, 0), M(1, 0, 0), M(0, 0, 0),
+              M(1, 1, 0), M(0, 1, 0), q);
+  W[6] += q[0]; W[3] += q[1]; W[7] += q[2]; W[4] += q[3]; W[8] += q[4]; W[5] += q[5];
+  minor_exact(d[1][2], M(1, 0, -1), M(0, 0, -1), M(1, 0, 0), M(0, 0, 0),
+              M(1, 0, 1), M(0, 0, 1), q);
+  W[16] += q[0]; W[11] += q[1]; W[7] += q[2]; W[4] += q[3]; W[18] += q[4]; W[12] += q[5];
+  // q (i, j-1/2, k)
+  qm = d[2][0] * (double)M(0, -1, 0);
+  W[3] += qm; W[4] -= qm;
+  minor_exact(d[2][1], M(-1, -1, 0), M(-1, 0, 0), M(0, -1, 0), M(0, 0, 0),
+              M(1, -1, 0), M(1, 0, 0), q);
+  W[0] -= q[0]; W[1] -= q[1]; W[3] -= q[2]; W[4] -= q[3]; W[6] -= q[4]; W[7] -= q[5];
+  minor_exact(d[2][2], M(0, -1, -1), M(0, 0, -1), M(0, -1, 0), M(0, 0, 0),
+              M(0, -1, 1), M(0, 0, 1), q);
+  W[9] -= q[0]; W[11] -= q[1]; W[3] -= q[2]; W[4] -= q[3]; W[10] -= q[4]; W[12] -= q[5];
+  // q (i, j+1/2, k)
+  qm = d[3][0] * (double)M(0, 1, 0);
+  W[5] += qm; W[4] -= qm;
+  minor_exact(d[3][1], M(-1, 1, 0), M(-1, 0, 0), M(0, 1, 0), M(0, 0, 0),
+              M(1, 1, 0), M(1, 0, 0), q);
+  W[2] += q[0]; W[1] += q[1]; W[5] += q[2]; W[4] += q[3]; W[8] += q[4]; W[7] += q[5];
+  minor_exact(d[3][2], M(0, 1, -1), M(0, 0, -1), M(0, 1, 0), M(0, 0, 0),
+              M(0, 1, 1), M(0, 0, 1), q);
+  W[13] += q[0]; W[11] += q[1]; W[5] += q[2]; W[4] += q[3]; W[14] += q[4]; W[12] += q[5];
+  // q (i, j, k-1/2)
+  qm = d[4][0] * (double)M(0, 0, -1);
+  W[11] += qm; W[4] -= qm;
+  minor_exact(d[4][1], M(-1, 0, -1), M(-1, 0, 0), M(0, 0, -1), M(0, 0, 0),
+              M(1, 0, -1), M(1, 0, 0), q);
+  W[15] -= q[0]; W[1] -= q[1]; W[11] -= q[2]; W[4] -= q[3]; W[16] -= q[4]; W[7] -= q[5];
+  minor_exact(d[4][2], M(0, -1, -1), M(0, -1, 0), M(0, 0, -1), M(0, 0, 0),
+              M(0, 1, -1), M(0, 1, 0), q);
+  W[9] -= q[0]; W[3] -= q[1]; W[11] -= q[2]; W[4] -= q[3]; W[13] -= q[4]; W[5] -= q[5];
+  // q (i, j, k+1/2)
+  qm = d[5][0] * (double)M(0, 0, 1);
+  W[12] += qm; W[4] -= qm;
+  minor_exact(d[5][1], M(-1, 0, 1), M(-1, 0, 0), M(0, 0, 1), M(0, 0, 0),
+              M(1, 0, 1), M(1, 0, 0), q);
+  W[17] += q[0]; W[1] += q[1]; W[12] += q[2]; W[4] += q[3]; W[18] += q[4]; W[7] += q[5];
+  minor_exact(d[5][2], M(0, -1, 1), M(0, -1, 0), M(0, 0, 1), M(0, 0, 0),
+              M(0, 1, 1), M(0, 1, 0), q);
+  W[10] += q[0]; W[3] += q[1]; W[12] += q[2]; W[4] += q[3]; W[14] += q[4]; W[5] += q[5];
+}
+
+// symmetric_stencil_2d.py:120-150 (weights builder only)
+__device__ __forceinline__ void sym_comp(double dxx, double dxy, double dyx,
+                                         double dyy, int m0, int m1, int m2,
+                                         int m3, int qx, int qy,
+                                         double out[4]) {
+  const int msum = m0 + m1 + m2 + m3;
+  if (msum < 3) {
+    out[0] = out[1] = out[2] = out[3] = 0.0;
+    return;
+  }
+  const double qdx = (double)qx * dxx + (double)qy * dyx;
+  const double qdy = (double)qx * dxy + (double)qy * dyy;
+  const double w0 = (double)(-m0) / (double)(m0 + m1) * qdx -
+                    (double)m0 / (double)(m0 + m2) * qdy;
+  const double w1 = (double)(-m1) / (double)(m0 + m1) * qdx +
+                    (double)m1 / (double)(m1 + m3) * qdy;
+  const double w2 = (double)m2 / (double)(m2 + m3) * qdx -
+                    (double)m2 / (double)(m0 + m2) * qdy;
+  const double w3 = (double)m3 / (double)(m2 + m3) * qdx +
+                    (double)m3 / (double)(m1 + m3) * qdy;
+  out[0] = 0.5 * w0;
+  out[1] = 0.5 * w1;
+  out[2] = 0.5 * w2;
+  out[3] = 0.5 * w3;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 flux form
+// ---------------------------------------------------------------------------
+
+// d * (mean of conducting higher-side nodes - mean of lower-side nodes); the
+// algebraic value of sum_k q_k u_k for the minor weights q of the reference.
+template <typename R>
+__device__ __forceinline__ R minor_flux(R d, int m0, R u0, int m1, R u1, int m2,
+                                        R u2, int m3, R u3, int m4, R u4,
+                                        int m5, R u5) {
+  const int mh = m2 + m3 + m4 + m5;
+  const int ml = m0 + m1 + m2 + m3;
+  if (m2 == 0 || m3 == 0 || mh < 3 || ml < 3) return R(0);
+  const R common = u2 + u3;
+  const R sh = common + (m4 ? u4 : R(0)) + (m5 ? u5 : R(0));
+  const R sl = common + (m0 ? u0 : R(0)) + (m1 ? u1 : R(0));
+  const R ih = mh == 4 ? R(0.25) : R(1.0 / 3.0);
+  const R il = ml == 4 ? R(0.25) : R(1.0 / 3.0);
+  return d * (sh * ih - sl * il);
+}
+
+// ---------------------------------------------------------------------------
+// The fused step
+// ---------------------------------------------------------------------------
+
+template <typename R, int MODEL>
+__device__ __forceinline__ double state_as_double(const StatePtrs& S, int k,
+                                                  int64_t c, int prec) {
+  if (state_is_f64(MODEL, prec, k)) return static_cast<const double*>(S.p[k])[c];
+  return (double)static_cast<const float*>(S.p[k])[c];
+}
+
+template <int NDIM, int ST, typename R>
+__device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
+                                     int64_t k, int64_t c, R uc) {
+  const int64_t si = a.nj * a.nk, sj = a.nk;  // 2D: nk = 1, sj = 1, si = nj
+  const R* __restrict__ u = a.u;
+  if constexpr (NDIM == 2) {
+    const int64_t sN = si;  // i stride
+    if constexpr (ST == 2) {
+      // precomputed (*shape, K) weights summed in the reference kernel's
+      // order; K (5 or 9) travels in a.f[0].mode
+      const int K = a.f[0].mode;
+      const double* W = a.w + c * K;
+      if (K == 5)
+        return (R)((double)u[c - sN] * W[0] + (double)u[c - 1] * W[1] +
+                   (double)uc * W[2] + (double)u[c + 1] * W[3] +
+                   (double)u[c + sN] * W[4]);
+      return (R)((double)u[c - sN - 1] * W[0] + (double)u[c - sN] * W[1] +
+                 (double)u[c - sN + 1] * W[2] + (double)u[c - 1] * W[3] +
+                 (double)uc * W[4] + (double)u[c + 1] * W[5] +
+                 (double)u[c + sN - 1] * W[6] + (double)u[c + sN] * W[7] +
+                 (double)u[c + sN + 1] * W[8]);
+    } else if constexpr (ST == 0) {
+      const int mN = a.code[c - sN] & 1, mS = a.code[c + sN] & 1;
+      const int mW = a.code[c - 1] & 1, mE = a.code[c + 1] & 1;
+      const double dN = fval(a.f[0], i - 1, j, 0, c - sN);
+      const double dS = fval(a.f[0], i, j, 0, c);
+      const double dW = fval(a.f[1], i, j - 1, 0, c - 1);
+      const double dE = fval(a.f[1], i, j, 0, c);
+      if constexpr (sizeof(R) == 8) {
+        double W[5];
+        W[0] = iso_comp(dN, mN, mS);
+        W[1] = iso_comp(dW, mW, mE);
+        W[3] = iso_comp(dE, mE, mW);
+        W[4] = iso_comp(dS, mS, mN);
+        W[2] = -(W[0] + W[1] + W[3] + W[4]);
+        scale_exact<5>(W, 2, a.D, a.dtd, a.dr2);
+        return u[c - sN] * W[0] + u[c - 1] * W[1] + uc * W[2] + u[c + 1] * W[3] +
+               u[c + sN] * W[4];
+      } else {
+        R fl = R(0);
+        if (mN) fl += (R)dN * (mS ? R(1) : R(2)) * (u[c - sN] - uc);
+        if (mW) fl += (R)dW * (mE ? R(1) : R(2)) * (u[c - 1] - uc);
+        if (mE) fl += (R)dE * (mW ? R(1) : R(2)) * (u[c + 1] - uc);
+        if (mS) fl += (R)dS * (mN ? R(1) : R(2)) * (u[c + sN] - uc);
+        return uc + a.s * fl;
+      }
+    } else {  // anisotropic 9-point
+      Nbr M;
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+#pragma unroll
+          for (int dk = -1; dk <= 1; ++dk) M.m[(di + 1) * 9 + (dj + 1) * 3 + dk + 1] = 0;
+          M.m[(di + 1) * 9 + (dj + 1) * 3 + 1] = a.code[c + di * sN + dj] & 1;
+        }
+      const double dxx0 = fval(a.f[0], i - 1, j, 0, c - sN);
+      const double dxx1 = fval(a.f[0], i, j, 0, c);
+      const double dxy0 = fval(a.f[1], i - 1, j, 0, c - sN);
+      const double dxy1 = fval(a.f[1], i, j, 0, c);
+      const double dyx0 = fval(a.f[2], i, j - 1, 0, c - 1);
+      const double dyx1 = fval(a.f[2], i, j, 0, c);
+      const double dyy0 = fval(a.f[3], i, j - 1, 0, c - 1);
+      const double dyy1 = fval(a.f[3], i, j, 0, c);
+      if constexpr (sizeof(R) == 8) {
+        double W[9];
+        wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
+        scale_exact<9>(W, 4, a.D, a.dtd, a.dr2);
+        return u[c - sN - 1] * W[0] + u[c - sN] * W[1] + u[c - sN + 1] * W[2] +
+               u[c - 1] * W[3] + uc * W[4] + u[c + 1] * W[5] +
+               u[c + sN - 1] * W[6] + u[c + sN] * W[7] + u[c + sN + 1] * W[8];
+      } else {
+        const R uNW = u[c - sN - 1], uN = u[c - sN], uNE = u[c - sN + 1];
+        const R uW = u[c - 1], uE = u[c + 1];
+        const R uSW = u[c + sN - 1], uS = u[c + sN], uSE = u[c + sN + 1];
+        const int mNW = M(-1, -1, 0), mN = M(-1, 0, 0), mNE = M(-1, 1, 0);
+        const int mW = M(0, -1, 0), mE = M(0, 1, 0);
+        const int mSW = M(1, -1, 0), mS = M(1, 0, 0), mSE = M(1, 1, 0);
+        R fl = R(0);
+        // face x0 (i-1/2): major, then the minor with sign -
+        if (mN) fl += (R)dxx0 * (uN - uc);
+        fl -= minor_flux<R>((R)dxy0, mNW, uNW, mW, uW, mN, uN, 1, uc, mNE, uNE, mE, uE);
+        // face y0 (j-1/2)
+        if (mW) fl += (R)dyy0 * (uW - uc);
+        fl -= minor_flux<R>((R)dyx0, mNW, uNW, mN, uN, mW, uW, 1, uc, mSW, uSW, mS, uS);
+        // face y1 (j+1/2)
+        if (mE) fl += (R)dyy1 * (uE - uc);
+        fl += minor_flux<R>((R)dyx1, mNE, uNE, mN, uN, mE, uE, 1, uc, mSE, uSE, mS, uS);
+        // face x1 (i+1/2)
+        if (mS) fl += (R)dxx1 * (uS - uc);
+        fl += minor_flux<R>((R)dxy1, mSW, uSW, mW, uW, mS, uS, 1, uc, mSE, uSE, mE, uE);
+        return uc + a.s * fl;
+      }
+    }
+  } else {  // NDIM == 3
+    if constexpr (ST == 2) {
+      const int K = a.f[0].mode;
+      const double* W = a.w + c * K;
+      if (K == 7)
+        return (R)((double)u[c - si] * W[0] + (double)u[c - sj] * W[1] +
+                   (double)u[c - 1] * W[2] + (double)uc * W[3] +
+                   (double)u[c + 1] * W[4] + (double)u[c + sj] * W[5] +
+                   (double)u[c + si] * W[6]);
+      return (R)((double)u[c - si - sj] * W[0] + (double)u[c - si] * W[1] +
+                 (double)u[c - si + sj] * W[2] + (double)u[c - sj] * W[3] +
+                 (double)uc * W[4] + (double)u[c + sj] * W[5] +
+                 (double)u[c + si - sj] * W[6] + (double)u[c + si] * W[7] +
+                 (double)u[c + si + sj] * W[8] + (double)u[c - sj - 1] * W[9] +
+                 (double)u[c - sj + 1] * W[10] + (double)u[c - 1] * W[11] +
+                 (double)u[c + 1] * W[12] + (double)u[c + sj - 1] * W[13] +
+                 (double)u[c + sj + 1] * W[14] + (double)u[c - si - 1] * W[15] +
+                 (double)u[c + si - 1] * W[16] + (double)u[c - si + 1] * W[17] +
+                 (double)u[c + si + 1] * W[18]);
+    } else if constexpr (ST == 0) {
+      const int mXm = a.code[c - si] & 1, mXp = a.code[c + si] & 1;
+      const int mYm = a.code[c - sj] & 1, mYp = a.code[c + sj] & 1;
+      const int mZm = a.code[c - 1] & 1, mZp = a.code[c + 1] & 1;
+      const double dXm = fval(a.f[0], i - 1, j, k, c - si);
+      const double dXp = fval(a.f[0], i, j, k, c);
+      const double dYm = fval(a.f[1], i, j - 1, k, c - sj);
+      const double dYp = fval(a.f[1], i, j, k, c);
+      const double dZm = fval(a.f[2], i, j, k - 1, c - 1);
+      const double dZp = fval(a.f[2], i, j, k, c);
+      if constexpr (sizeof(R) == 8) {
+        double W[7];
+        W[0] = iso_comp(dXm, mXm, mXp);
+        W[1] = iso_comp(dYm, mYm, mYp);
+        W[2] = iso_comp(dZm, mZm, mZp);
+        W[4] = iso_comp(dZp, mZp, mZm);
+        W[5] = iso_comp(dYp, mYp, mYm);
+        W[6] = iso_comp(dXp, mXp, mXm);
+        W[3] = -(W[0] + W[1] + W[2] + W[4] + W[5] + W[6]);
+        scale_exact<7>(W, 3, a.D, a.dtd, a.dr2);
+        return u[c - si] * W[0] + u[c - sj] * W[1] + u[c - 1] * W[2] + uc * W[3] +
+               u[c + 1] * W[4] + u[c + sj] * W[5] + u[c + si] * W[6];
+      } else {
+        R fl = R(0);
+        if (mXm) fl += (R)dXm * (mXp ? R(1) : R(2)) * (u[c - si] - uc);
+        if (mYm) fl += (R)dYm * (mYp ? R(1) : R(2)) * (u[c - sj] - uc);
+        if (mZm) fl += (R)dZm * (mZp ? R(1) : R(2)) * (u[c - 1] - uc);
+        if (mZp) fl += (R)dZp * (mZm ? R(1) : R(2)) * (u[c + 1] - uc);
+        if (mYp) fl += (R)dYp * (mYm ? R(1) : R(2)) * (u[c + sj] - uc);
+        if (mXp) fl += (R)dXp * (mXm ? R(1) : R(2)) * (u[c + si] - uc);
+        return uc + a.s * fl;
+      }
+    } else {  // anisotropic 19-point
+      Nbr M;
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+          for (int dk = -1; dk <= 1; ++dk) {
+            const int corner = (di != 0) && (dj != 0) && (dk != 0);
+            M.m[(di + 1) * 9 + (dj + 1) * 3 + dk + 1] =
+                corner ? 0 : (a.code[c + di * si + dj * sj + dk] & 1);
+          }
+      // fields: 0 xx 1 xy 2 xz 3 yx 4 yy 5 yz 6 zx 7 zy 8 zz
+      double d[6][3];
+      d[0][0] = fval(a.f[0], i - 1, j, k, c - si);
+      d[0][1] = fval(a.f[1], i - 1, j, k, c - si);
+      d[0][2] = fval(a.f[2], i - 1, j, k, c - si);
+      d[1][0] = fval(a.f[0], i, j, k, c);
+      d[1][1] = fval(a.f[1], i, j, k, c);
+      d[1][2] = fval(a.f[2], i, j, k, c);
+      d[2][0] = fval(a.f[4], i, j - 1, k, c - sj);
+      d[2][1] = fval(a.f[3], i, j - 1, k, c - sj);
+      d[2][2] = fval(a.f[5], i, j - 1, k, c - sj);
+      d[3][0] = fval(a.f[4], i, j, k, c);
+      d[3][1] = fval(a.f[3], i, j, k, c);
+      d[3][2] = fval(a.f[5], i, j, k, c);
+      d[4][0] = fval(a.f[8], i, j, k - 1, c - 1);
+      d[4][1] = fval(a.f[6], i, j, k - 1, c - 1);
+      d[4][2] = fval(a.f[7], i, j, k - 1, c - 1);
+      d[5][0] = fval(a.f[8], i, j, k, c);
+      d[5][1] = fval(a.f[6], i, j, k, c);
+      d[5][2] = fval(a.f[7], i, j, k, c);
+      if constexpr (sizeof(R) == 8) {
+        double W[19];
+        wts_aniso3d(W, M, d);
+        scale_exact<19>(W, 4, a.D, a.dtd, a.dr2);
+        return u[c - si - sj] * W[0] + u[c - si] * W[1] + u[c - si + sj] * W[2] +
+               u[c - sj] * W[3] + uc * W[4] + u[c + sj] * W[5] +
+               u[c + si - sj] * W[6] + u[c + si] * W[7] + u[c + si + sj] * W[8] +
+               u[c - sj - 1] * W[9] + u[c - sj + 1] * W[10] + u[c - 1] * W[11] +
+               u[c + 1] * W[12] + u[c + sj - 1] * W[13] + u[c + sj + 1] * W[14] +
+               u[c - si - 1] * W[15] + u[c + si - 1] * W[16] +
+               u[c - si + 1] * W[17] + u[c + si + 1] * W[18];
+      } else {
+        auto U = [&](int di, int dj, int dk) -> R {
+          return M(di, dj, dk) ? u[c + di * si + dj * sj + dk] : R(0);
+        };
+        R fl = R(0);
+        // x0
+        if (M(-1, 0, 0)) fl += (R)d[0][0] * (U(-1, 0, 0) - uc);
+        fl -= minor_flux<R>((R)d[0][1], M(-1, -1, 0), U(-1, -1, 0), M(0, -1, 0), U(0, -1, 0),
+                            M(-1, 0, 0), U(-1, 0, 0), 1, uc, M(-1, 1, 0), U(-1, 1, 0),
+                            M(0, 1, 0), U(0, 1, 0));
+        fl -= minor_flux<R>((R)d[0][2], M(-1, 0, -1), U(-1, 0, -1), M(0, 0, -1), U(0, 0, -1),
+                            M(-1, 0, 0), U(-1, 0, 0), 1, uc, M(-1, 0, 1), U(-1, 0, 1),
+                            M(0, 0, 1), U(0, 0, 1));
+        // x1
+        if (M(1, 0, 0)) fl += (R)d[1][0] * (U(1, 0, 0) - uc);
+        fl += minor_flux<R>((R)d[1][1], M(1, -1, 0), U(1, -1, 0), M(0, -1, 0), U(0, -1, 0),
+                            M(1, 0, 0), U(1, 0, 0), 1, uc, M(1, 1, 0), U(1, 1, 0),
+                            M(0, 1, 0), U(0, 1, 0));
+        fl += minor_flux<R>((R)d[1][2], M(1, 0, -1), U(1, 0, -1), M(0, 0, -1), U(0, 0, -1),
+                            M(1, 0, 0), U(1, 0, 0), 1, uc, M(1, 0, 1), U(1, 0, 1),
+                            M(0, 0, 1), U(0, 0, 1));
+        // y0
+        if (M(0, -1, 0)) fl += (R)d[2][0] * (U(0, -1, 0) - uc);
+        fl -= minor_flux<R>((R)d[2][1], M(-1, -1, 0), U(-1, -1, 0), M(-1, 0, 0), U(-1, 0, 0),
+                            M(0, -1, 0), U(0, -1, 0), 1, uc, M(1, -1, 0), U(1, -1, 0),
+                            M(1, 0, 0), U(1, 0, 0));
+        fl -= minor_flux<R>((R)d[2][2], M(0, -1, -1), U(0, -1, -1), M(0, 0, -1), U(0, 0, -1),
+                            M(0, -1, 0), U(0, -1, 0), 1, uc, M(0, -1, 1), U(0, -1, 1),
+                            M(0, 0, 1), U(0, 0, 1));
+        // y1
+        if (M(0, 1, 0)) fl += (R)d[3][0] * (U(0, 1, 0) - uc);
+        fl += minor_flux<R>((R)d[3][1], M(-1, 1, 0), U(-1, 1, 0), M(-1, 0, 0), U(-1, 0, 0),
+                            M(0, 1, 0), U(0, 1, 0), 1, uc, M(1, 1, 0), U(1, 1, 0),
+                            M(1, 0, 0), U(1, 0, 0));
+        fl += minor_flux<R>((R)d[3][2], M(0, 1, -1), U(0, 1, -1), M(0, 0, -1), U(0, 0, -1),
+                            M(0, 1, 0), U(0, 1, 0), 1, uc, M(0, 1, 1), U(0, 1, 1),
+                            M(0, 0, 1), U(0, 0, 1));
+        // z0
+        if (M(0, 0, -1)) fl += (R)d[4][0] * (U(0, 0, -1) - uc);
+        fl -= minor_flux<R>((R)d[4][1], M(-1, 0, -1), U(-1, 0, -1), M(-1, 0, 0), U(-1, 0, 0),
+                            M(0, 0, -1), U(0, 0, -1), 1, uc, M(1, 0, -1), U(1, 0, -1),
+                            M(1, 0, 0), U(1, 0, 0));
+        fl -= minor_flux<R>((R)d[4][2], M(0, -1, -1), U(0, -1, -1), M(0, -1, 0), U(0, -1, 0),
+                            M(0, 0, -1), U(0, 0, -1), 1, uc, M(0, 1, -1), U(0, 1, -1),
+                            M(0, 1, 0), U(0, 1, 0));
+        // z1
+        if (M(0, 0, 1)) fl += (R)d[5][0] * (U(0, 0, 1) - uc);
+        fl += minor_flux<R>((R)d[5][1], M(-1, 0, 1), U(-1, 0, 1), M(-1, 0, 0), U(-1, 0, 0),
+                            M(0, 0, 1), U(0, 0, 1), 1, uc, M(1, 0, 1), U(1, 0, 1),
+                            M(1, 0, 0), U(1, 0, 0));
+        fl += minor_flux<R>((R)d[5][2], M(0, -1, 1), U(0, -1, 1), M(0, -1, 0), U(0, -1, 0),
+                            M(0, 0, 1), U(0, 0, 1), 1, uc, M(0, 1, 1), U(0, 1, 1),
+                            M(0, 1, 0), U(0, 1, 0));
+        return uc + a.s * fl;
+      }
+    }
+  }
+}
+
+template <int NDIM, int ST, int MODEL, typename R, int PREC>
+__global__ __launch_bounds__(kBlock) void k_step(const KArgs<R> a) {
+  const int64_t lb = logical_block(a.per_xcd);
+  if (lb >= a.nblocks) return;
+  const int64_t c = lb * kBlock + threadIdx.x;
+  if (c >= a.N) return;
+  const uint8_t code = a.code[c];
+  const R uc = a.u[c];
+  const int32_t step = a.step_in ? *a.step_in : a.step;
+  if (a.act_on) {
+    if (a.act[c] < 0 && uc > a.act_thr) a.act[c] = step;
+  }
+  if ((code & 3) == 1) {
+    // node coordinates; skipped work for nodes on the outer layer keeps every
+    // neighbour access inside the grid whatever the codes say
+    const int64_t sjk = a.nj * a.nk;
+    const int64_t i = c / sjk;
+    const int64_t rem = c - i * sjk;
+    const int64_t j = rem / a.nk;
+    const int64_t k = rem - j * a.nk;
+    bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
+    if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
+    if (interior) {
+      R un = diffuse<NDIM, ST, R>(a, i, j, k, c, uc);
+      un = ionic_update<MODEL, R>(uc, un, a.st, c, a.dt, a.P);
+      a.u_new[c] = un;
+    }
+  }
+  if ((code & 4) && a.probe_row >= 0) {
+    for (int p = 0; p < a.n_probe; ++p) {
+      if (a.probe_idx[p] != c) continue;
+      const int var = a.probe_var[p];
+      const double val = var < 0 ? (double)uc : state_as_double<R, MODEL>(a.st, var, c, PREC);
+      a.probe_buf[(int64_t)a.probe_row * a.n_probe + p] = val;
+    }
+  }
+  if (a.step_out && c == 0) *a.step_out = step + 1;
+}
+
+// ---------------------------------------------------------------------------
+// Literal sweeps (one numba kernel each)
+// ---------------------------------------------------------------------------
+
+// compute_weights over all nodes; d arrays are full fields, m int8 0/1
+struct DPtrs {
+  const double* p[9];
+};
+
+template <int NDIM, int KIND>
+__global__ __launch_bounds__(kBlock) void k_weights(double* __restrict__ w,
+                                                    const int8_t* __restrict__ m,
+                                                    const DPtrs dp, int64_t ni,
+                                                    int64_t nj, int64_t nk) {
+  const double* const* d = dp.p;
+  const int64_t N = ni * nj * nk;
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= N) return;
+  if (m[c] != 1) return;
+  const int64_t sjk = nj * nk, si = sjk, sj = nk;
+  const int64_t i = c / sjk, j = (c - i * sjk) / nk, k = c - i * sjk - j * nk;
+  if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1) return;
+  if (NDIM == 3 && (k == 0 || k == nk - 1)) return;
+  if constexpr (NDIM == 2 && KIND == 0) {
+    const int mN = m[c - si], mS = m[c + si], mW = m[c - 1], mE = m[c + 1];
+    double* W = w + c * 5;
+    W[0] = iso_comp(d[0][c - si], mN, mS);
+    W[1] = iso_comp(d[1][c - 1], mW, mE);
+    W[3] = iso_comp(d[1][c], mE, mW);
+    W[4] = iso_comp(d[0][c], mS, mN);
+    W[2] = -(W[0] + W[1] + W[3] + W[4]);
+  } else if constexpr (NDIM == 3 && KIND == 0) {
+    double* W = w + c * 7;
+    W[0] = iso_comp(d[0][c - si], m[c - si], m[c + si]);
+    W[1] = iso_comp(d[1][c - sj], m[c - sj], m[c + sj]);
+    W[2] = iso_comp(d[2][c - 1], m[c - 1], m[c + 1]);
+    W[4] = iso_comp(d[2][c], m[c + 1], m[c - 1]);
+    W[5] = iso_comp(d[1][c], m[c + sj], m[c - sj]);
+    W[6] = iso_comp(d[0][c], m[c + si], m[c - si]);
+    W[3] = -(W[0] + W[1] + W[2] + W[4] + W[5] + W[6]);
+  } else if constexpr (NDIM == 2 && KIND == 1) {
+    Nbr M;
+    for (int q = 0; q < 27; ++q) M.m[q] = 0;
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj) M.m[(di + 1) * 9 + (dj + 1) * 3 + 1] = m[c + di * si + dj];
+    double W[9];
+    wts_aniso2d(W, M, d[0][c - si], d[0][c], d[1][c - si], d[1][c], d[2][c - 1], d[2][c],
+                d[3][c - 1], d[3][c]);
+    for (int q = 0; q < 9; ++q) w[c * 9 + q] = W[q];
+  } else if constexpr (NDIM == 3 && KIND == 1) {
+    Nbr M;
+    for (int di = -1; di <= 1; ++di)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int dk = -1; dk <= 1; ++dk) {
+          const int corner = (di != 0) && (dj != 0) && (dk != 0);
+          M.m[(di + 1) * 9 + (dj + 1) * 3 + dk + 1] =
+              corner ? 0 : m[c + di * si + dj * sj + dk];
+        }
+    double dd[6][3];
+    dd[0][0] = d[0][c - si]; dd[0][1] = d[1][c - si]; dd[0][2] = d[2][c - si];
+    dd[1][0] = d[0][c];      dd[1][1] = d[1][c];      dd[1][2] = d[2][c];
+    dd[2][0] = d[4][c - sj]; dd[2][1] = d[3][c - sj]; dd[2][2] = d[5][c - sj];
+    dd[3][0] = d[4][c];      dd[3][1] = d[3][c];      dd[3][2] = d[5][c];
+    dd[4][0] = d[8][c - 1];  dd[4][1] = d[6][c - 1];  dd[4][2] = d[7][c - 1];
+    dd[5][0] = d[8][c];      dd[5][1] = d[6][c];      dd[5][2] = d[7][c];
+    double W[19];
+    wts_aniso3d(W, M, dd);
+    for (int q = 0; q < 19; ++q) w[c * 19 + q] = W[q];
+  } else if constexpr (NDIM == 2 && KIND == 3) {
+    // symmetric_stencil_2d.py:159-250, corner fields at (i-1/2, j-1/2) etc.
+    double* W = w + c * 9;
+    double o[4];
+    int64_t p = c - si - 1;
+    sym_comp(d[0][p], d[1][p], d[2][p], d[3][p], m[c - si - 1], m[c - si], m[c - 1], m[c], -1,
+             -1, o);
+    W[0] += o[0]; W[1] += o[1]; W[3] += o[2]; W[4] += o[3];
+    p = c - si;
+    sym_comp(d[0][p], d[1][p], d[2][p], d[3][p], m[c - si], m[c - si + 1], m[c], m[c + 1], -1,
+             1, o);
+    W[1] += o[0]; W[2] += o[1]; W[4] += o[2]; W[5] += o[3];
+    p = c - 1;
+    sym_comp(d[0][p], d[1][p], d[2][p], d[3][p], m[c - 1], m[c], m[c + si - 1], m[c + si], 1,
+             -1, o);
+    W[3] += o[0]; W[4] += o[1]; W[6] += o[2]; W[7] += o[3];
+    p = c;
+    sym_comp(d[0][p], d[1][p], d[2][p], d[3][p], m[c], m[c + 1], m[c + si], m[c + si + 1], 1,
+             1, o);
+    W[4] += o[0]; W[5] += o[1]; W[7] += o[2]; W[8] += o[3];
+  }
+}
+
+template <int NDIM, int K>
+__global__ __launch_bounds__(kBlock) void k_diffusion(double* __restrict__ u_new,
+                                                      const double* __restrict__ u,
+                                                      const double* __restrict__ w,
+                                                      const int64_t* __restrict__ idx,
+                                                      int64_t n, int64_t nj, int64_t nk) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  const int64_t c = idx[q];
+  const double* W = w + c * K;
+  if constexpr (NDIM == 2 && K == 5) {
+    u_new[c] = u[c - nj] * W[0] + u[c - 1] * W[1] + u[c] * W[2] + u[c + 1] * W[3] +
+               u[c + nj] * W[4];
+  } else if constexpr (NDIM == 2 && K == 9) {
+    u_new[c] = u[c - nj - 1] * W[0] + u[c - nj] * W[1] + u[c - nj + 1] * W[2] +
+               u[c - 1] * W[3] + u[c] * W[4] + u[c + 1] * W[5] + u[c + nj - 1] * W[6] +
+               u[c + nj] * W[7] + u[c + nj + 1] * W[8];
+  } else if constexpr (NDIM == 3 && K == 7) {
+    const int64_t si = nj * nk, sj = nk;
+    u_new[c] = u[c - si] * W[0] + u[c - sj] * W[1] + u[c - 1] * W[2] + u[c] * W[3] +
+               u[c + 1] * W[4] + u[c + sj] * W[5] + u[c + si] * W[6];
+  } else {
+    const int64_t si = nj * nk, sj = nk;
+    u_new[c] = u[c - si - sj] * W[0] + u[c - si] * W[1] + u[c - si + sj] * W[2] +
+               u[c - sj] * W[3] + u[c] * W[4] + u[c + sj] * W[5] + u[c + si - sj] * W[6] +
+               u[c + si] * W[7] + u[c + si + sj] * W[8] + u[c - sj - 1] * W[9] +
+               u[c - sj + 1] * W[10] + u[c - 1] * W[11] + u[c + 1] * W[12] +
+               u[c + sj - 1] * W[13] + u[c + sj + 1] * W[14] + u[c - si - 1] * W[15] +
+               u[c + si - 1] * W[16] + u[c - si + 1] * W[17] + u[c + si + 1] * W[18];
+  }
+}
+
+template <int MODEL, typename R>
+__global__ __launch_bounds__(kBlock) void k_ionic(R* __restrict__ u_new,
+                                                  const R* __restrict__ u,
+                                                  const StatePtrs st,
+                                                  const int64_t* __restrict__ idx,
+                                                  int64_t n, R dt, const Params P) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  const int64_t c = idx[q];
+  u_new[c] = ionic_update<MODEL, R>(u[c], u_new[c], st, c, dt, P);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_stim(R* __restrict__ u,
+                                                 const uint8_t* __restrict__ code,
+                                                 const uint8_t* __restrict__ mask,
+                                                 int64_t lo0, int64_t lo1, int64_t lo2,
+                                                 int64_t b0, int64_t b1, int64_t b2,
+                                                 int64_t nj, int64_t nk, int kind,
+                                                 double value, int clamp, double u_max) {
+  const int64_t n = b0 * b1 * b2;
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (q >= n) return;
+  const int64_t a0 = q / (b1 * b2), r = q - a0 * b1 * b2, a1 = r / b2, a2 = r - a1 * b2;
+  const int64_t c = ((lo0 + a0) * nj + (lo1 + a1)) * nk + (lo2 + a2);
+  if ((code[c] & 1) == 0) return;
+  if (mask && mask[c] == 0) return;
+  if (kind == 0) {
+    u[c] = (R)value;
+  } else {
+    R v = u[c] + (R)value;
+    if (clamp && v > (R)u_max) v = (R)u_max;
+    u[c] = v;
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_act(int32_t* __restrict__ act,
+                                                const R* __restrict__ u, int64_t n,
+                                                R thr, int32_t step) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= n) return;
+  if (act[c] < 0 && u[c] > thr) act[c] = step;
+}
+
+}  // namespace fw

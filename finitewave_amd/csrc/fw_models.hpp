@@ -1,0 +1,777 @@
+// fw_models.hpp -- per-cell ionic updates for gfx950, templated on the
+// storage/arithmetic type R (double for the FP64 path, float for the FP32
+// path).  Calcium and ion concentrations of TP06 are always double.
+//
+// Each update reads the node's pre-step potential `u` and its state, writes
+// the state in place and returns the new u_new given the diffusion result
+// `un` -- the reference's "ionic_kernel after diffusion_kernel" order.
+// Operation order follows the reference statements (paths relative to the
+// finitewave source); in the FP64 translation unit, built with
+// -ffp-contract=off, the arithmetic is the reference's bit for bit except
+// for the libm implementation of exp/log/tanh/pow.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fw {
+
+constexpr int kBlock = 256;  // work-items per block of the node-parallel kernels
+constexpr int kXcds = 8;     // accelerator complex dies of an MI355X
+
+struct Params {
+  double v[64];
+};
+
+struct StatePtrs {
+  void* p[24];
+};
+
+// ---- math dispatch --------------------------------------------------------
+__device__ __forceinline__ double fexp(double x) { return exp(x); }
+__device__ __forceinline__ float fexp(float x) { return expf(x); }
+__device__ __forceinline__ double flog(double x) { return log(x); }
+__device__ __forceinline__ float flog(float x) { return logf(x); }
+__device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double ftanh(double x) { return tanh(x); }
+__device__ __forceinline__ float ftanh(float x) { return tanhf(x); }
+__device__ __forceinline__ double fpow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float fpow(float x, float y) { return powf(x, y); }
+
+template <typename T>
+__device__ __forceinline__ T& ref(const StatePtrs& S, int k, int64_t c) {
+  return static_cast<T*>(S.p[k])[c];
+}
+
+// ---- Aliev-Panfilov  aliev_panfilov_2d.py:133-203 ---------------------------
+template <typename R>
+__device__ __forceinline__ R ionic_ap(R u, R un, const StatePtrs& S, int64_t c,
+                                      R dt, const Params& P) {
+  const R a = (R)P.v[0], k = (R)P.v[1], eap = (R)P.v[2], mu_1 = (R)P.v[3],
+          mu_2 = (R)P.v[4];
+  R& vref = ref<R>(S, 0, c);
+  R v = vref;
+  // calc_v: v += (-dt*(eap + (mu_1*v)/(mu_2 + u)) * (v + k*u*(u - a - 1.)))
+  v += (-dt * (eap + (mu_1 * v) / (mu_2 + u)) * (v + k * u * (u - a - R(1))));
+  vref = v;
+  // u_new += dt*(-k*u*(u - a)*(u - 1.) - u*v)
+  return un + dt * (-k * u * (u - a) * (u - R(1)) - u * v);
+}
+
+// ---- Barkley  barkley_2d.py:113-166 ----------------------------------------
+template <typename R>
+__device__ __forceinline__ R ionic_barkley(R u, R un, const StatePtrs& S,
+                                           int64_t c, R dt, const Params& P) {
+  const R a = (R)P.v[0], b = (R)P.v[1], eap = (R)P.v[2];
+  R& vref = ref<R>(S, 0, c);
+  R v = vref;
+  v += dt * (u - v);
+  vref = v;
+  return un + dt * (u * (R(1) - u) * (u - (v + b) / a)) / eap;
+}
+
+// ---- Mitchell-Schaeffer  mitchell_schaeffer_2d.py:102-217 ------------------
+template <typename R>
+__device__ __forceinline__ R ionic_ms(R u, R un, const StatePtrs& S, int64_t c,
+                                      R dt, const Params& P) {
+  const R tau_close = (R)P.v[0], tau_open = (R)P.v[1], tau_in = (R)P.v[2],
+          tau_out = (R)P.v[3], u_gate = (R)P.v[4];
+  R& href = ref<R>(S, 0, c);
+  R h = href;
+  h += (u < u_gate) ? dt * (R(1) - h) / tau_open : dt * (-h) / tau_close;
+  href = h;
+  R C = (u * u) * (R(1) - u);  // (u**2)*(1-u), u**2 -> u*u
+  R J_in = h * C / tau_in;
+  R J_out = -u / tau_out;
+  return un + dt * (J_in + J_out);
+}
+
+// ---- Fenton-Karma  fenton_karma_2d.py:143-328 ------------------------------
+template <typename R>
+__device__ __forceinline__ R ionic_fk(R u, R un, const StatePtrs& S, int64_t c,
+                                      R dt, const Params& P) {
+  const R tau_d = (R)P.v[0], tau_o = (R)P.v[1], tau_r = (R)P.v[2],
+          tau_si = (R)P.v[3], tau_v_m = (R)P.v[4], tau_v_p = (R)P.v[5],
+          tau_w_m = (R)P.v[6], tau_w_p = (R)P.v[7], k = (R)P.v[8],
+          u_c = (R)P.v[9], uc_si = (R)P.v[10];
+  R& vref = ref<R>(S, 0, c);
+  R& wref = ref<R>(S, 1, c);
+  R v = vref, w = wref;
+  const R H1 = (u_c - u) >= 0 ? R(1) : R(0);
+  const R H2 = (u - u_c) >= 0 ? R(1) : R(0);
+  v += dt * (H1 * (R(1) - v) / tau_v_m - H2 * v / tau_v_p);
+  w += dt * (H1 * (R(1) - w) / tau_w_m - H2 * w / tau_w_p);
+  vref = v;
+  wref = w;
+  const R J_fi = -(v * H2 * (R(1) - u) * (u - u_c)) / tau_d;
+  const R J_so = u * H1 / tau_o + H2 / tau_r;
+  // calc_Jsi receives v (fenton_karma_2d.py:326)
+  const R J_si = -v * (R(1) + ftanh(k * (u - uc_si))) / (R(2) * tau_si);
+  return un + dt * (-J_fi - J_so - J_si);
+}
+
+// ---- Bueno-Orovio  bueno_orovio_2d.py:183-476 -------------------------------
+template <typename R>
+__device__ __forceinline__ R ionic_bo(R u, R un, const StatePtrs& S, int64_t c,
+                                      R dt, const Params& P) {
+  const R u_o = (R)P.v[0], u_u = (R)P.v[1], theta_v = (R)P.v[2],
+          theta_w = (R)P.v[3], theta_v_m = (R)P.v[4], theta_o = (R)P.v[5],
+          tau_v1_m = (R)P.v[6], tau_v2_m = (R)P.v[7], tau_v_p = (R)P.v[8],
+          tau_w1_m = (R)P.v[9], tau_w2_m = (R)P.v[10], k_w_m = (R)P.v[11],
+          u_w_m = (R)P.v[12], tau_w_p = (R)P.v[13], tau_fi = (R)P.v[14],
+          tau_o1 = (R)P.v[15], tau_o2 = (R)P.v[16], tau_so1 = (R)P.v[17],
+          tau_so2 = (R)P.v[18], k_so = (R)P.v[19], u_so = (R)P.v[20],
+          tau_s1 = (R)P.v[21], tau_s2 = (R)P.v[22], k_s = (R)P.v[23],
+          u_s = (R)P.v[24], tau_si = (R)P.v[25], tau_w_inf = (R)P.v[26],
+          w_inf_ = (R)P.v[27];
+  R& vref = ref<R>(S, 0, c);
+  R& wref = ref<R>(S, 1, c);
+  R& sref = ref<R>(S, 2, c);
+  R v = vref, w = wref, s = sref;
+  // calc_v(v, u, dt, theta_v, calc_v_inf(u), calc_tau_v_m(u), tau_v_p)
+  {
+    const R v_inf = u < theta_v_m ? R(1) : R(0);
+    const R tau_v_m = (u - theta_v_m) < 0 ? tau_v1_m : tau_v2_m;
+    const R v_ = (u - theta_v) < 0 ? (v_inf - v) / tau_v_m : -v / tau_v_p;
+    v += dt * v_;
+  }
+  // calc_w(w, u, dt, theta_w, calc_w_inf(u), calc_tau_w_m(u), tau_w_p)
+  {
+    const R w_inf = (u - theta_o) < 0 ? R(1) - u / tau_w_inf : w_inf_;
+    const R tau_w_m = tau_w1_m + (tau_w2_m - tau_w1_m) *
+                                     (R(1) + ftanh(k_w_m * (u - u_w_m))) / R(2);
+    const R w_ = (u - theta_w) < 0 ? (w_inf - w) / tau_w_m : -w / tau_w_p;
+    w += dt * w_;
+  }
+  // calc_s(s, u, dt, calc_tau_s(u), k_s, u_s)
+  {
+    const R tau_s = (u - theta_w) < 0 ? tau_s1 : tau_s2;
+    s += dt * ((R(1) + ftanh(k_s * (u - u_s))) / R(2) - s) / tau_s;
+  }
+  vref = v;
+  wref = w;
+  sref = s;
+  const R Hv = (u - theta_v) >= 0 ? R(1) : R(0);
+  const R J_fi = -v * Hv * (u - theta_v) * (u_u - u) / tau_fi;
+  const R tau_o = (u - theta_o) < 0 ? tau_o1 : tau_o2;
+  const R tau_so =
+      tau_so1 + (tau_so2 - tau_so1) * (R(1) + ftanh(k_so * (u - u_so))) / R(2);
+  const R Hw = (u - theta_w) >= 0 ? R(1) : R(0);
+  const R J_so = (u - u_o) * (R(1) - Hw) / tau_o + Hw / tau_so;
+  const R J_si = -Hw * w * s / tau_si;
+  return un + dt * (-J_fi - J_so - J_si);
+}
+
+// ---- Luo-Rudy 1991  luo_rudy91_2d.py:158-510 --------------------------------
+template <typename R>
+__device__ __forceinline__ R lr91_gate(R var, R dt, R alpha, R beta) {
+  const R tau = R(1) / (alpha + beta);
+  const R inf = alpha / (alpha + beta);
+  var += dt * (inf - var) / tau;
+  return var;
+}
+
+template <typename R>
+__device__ __forceinline__ R ionic_lr91(R u, R un, const StatePtrs& S,
+                                        int64_t c, R dt, const Params& P) {
+  const R gna = (R)P.v[0], gsi = (R)P.v[1], gk = (R)P.v[2], gk1 = (R)P.v[3],
+          gkp = (R)P.v[4], gb = (R)P.v[5], ko = (R)P.v[6], ki = (R)P.v[7],
+          nai = (R)P.v[8], nao = (R)P.v[9], R_ = (R)P.v[11], T = (R)P.v[12],
+          F = (R)P.v[13], PR_NaK = (R)P.v[14];
+  // E_Na = R*T/F*np.log(nao/nai) (computed once per launch in the reference)
+  const R E_Na = R_ * T / F * flog(nao / nai);
+  R& mr = ref<R>(S, 0, c);
+  R& hr = ref<R>(S, 1, c);
+  R& jr = ref<R>(S, 2, c);
+  R& dr = ref<R>(S, 3, c);
+  R& fr = ref<R>(S, 4, c);
+  R& xr = ref<R>(S, 5, c);
+  R& cair = ref<R>(S, 6, c);
+  // calc_ina
+  R ina;
+  {
+    R alpha_h = 0, beta_h = 0, beta_J = 0, alpha_J = 0;
+    if (u >= R(-40.0)) {
+      beta_h = R(1.0) / (R(0.13) * (R(1) + fexp((u + R(10.66)) / R(-11.1))));
+      beta_J = R(0.3) * fexp(R(-2.535 * 1e-07) * u) /
+               (R(1) + fexp(R(-0.1) * (u + R(32))));
+    } else {
+      alpha_h = R(0.135) * fexp((R(80) + u) / R(-6.8));
+      beta_h = R(3.56) * fexp(R(0.079) * u) +
+               R(3.1 * 100000.0) * fexp(R(0.35) * u);
+      beta_J = R(0.1212) * fexp(R(-0.01052) * u) /
+               (R(1) + fexp(R(-0.1378) * (u + R(40.14))));
+      alpha_J = (R(-1.2714 * 100000.0) * fexp(R(0.2444) * u) -
+                 R(3.474 * 1e-05) * fexp(R(-0.04391) * u)) *
+                (u + R(37.78)) / (R(1) + fexp(R(0.311) * (u + R(79.23))));
+    }
+    const R alpha_m =
+        R(0.32) * (u + R(47.13)) / (R(1) - fexp(R(-0.1) * (u + R(47.13))));
+    const R beta_m = R(0.08) * fexp(-u / R(11));
+    const R m = lr91_gate(mr, dt, alpha_m, beta_m);
+    const R h = lr91_gate(hr, dt, alpha_h, beta_h);
+    const R j = lr91_gate(jr, dt, alpha_J, beta_J);
+    mr = m;
+    hr = h;
+    jr = j;
+    ina = gna * m * m * m * h * j * (u - E_Na);
+  }
+  // calc_isk: the current uses the old d, f
+  R isi;
+  {
+    R d = dr, f = fr, cai = cair;
+    const R E_Si = R(7.7) - R(13.0287) * flog(cai);
+    isi = gsi * d * f * (u - E_Si);
+    const R alpha_d = R(0.095) * fexp(R(-0.01) * (u - R(5))) /
+                      (R(1) + fexp(R(-0.072) * (u - R(5))));
+    const R beta_d = R(0.07) * fexp(R(-0.017) * (u + R(44))) /
+                     (R(1) + fexp(R(0.05) * (u + R(44))));
+    const R alpha_f = R(0.012) * fexp(R(-0.008) * (u + R(28))) /
+                      (R(1) + fexp(R(0.15) * (u + R(28))));
+    const R beta_f = R(0.0065) * fexp(R(-0.02) * (u + R(30))) /
+                     (R(1) + fexp(R(-0.2) * (u + R(30))));
+    d = lr91_gate(d, dt, alpha_d, beta_d);
+    f = lr91_gate(f, dt, alpha_f, beta_f);
+    cai += dt * (R(-0.0001) * isi + R(0.07) * (R(0.0001) - cai));
+    dr = d;
+    fr = f;
+    cair = cai;
+  }
+  // calc_ik: the current uses the old x
+  R ik;
+  {
+    const R E_K = R_ * T / F * flog((ko + PR_NaK * nao) / (ki + PR_NaK * nai));
+    const R G_K = gk * fsqrt(ko / R(5.4));
+    R Xi;
+    if (u > R(-100)) {
+      Xi = R(2.837) * (fexp(R(0.04) * (u + R(77))) - R(1)) /
+           ((u + R(77)) * fexp(R(0.04) * (u + R(35))));
+    } else {
+      Xi = R(1);
+    }
+    const R x = xr;
+    ik = G_K * x * Xi * (u - E_K);
+    const R alpha_x = R(0.0005) * fexp(R(0.083) * (u + R(50))) /
+                      (R(1) + fexp(R(0.057) * (u + R(50))));
+    const R beta_x = R(0.0013) * fexp(R(-0.06) * (u + R(20))) /
+                     (R(1) + fexp(R(-0.04) * (u + R(20))));
+    xr = lr91_gate(x, dt, alpha_x, beta_x);
+  }
+  const R E_K1 = R_ * T / F * flog(ko / ki);
+  // calc_ik1
+  const R alpha_K1 = R(1.02) / (R(1) + fexp(R(0.2385) * (u - E_K1 - R(59.215))));
+  const R beta_K1 = (R(0.49124) * fexp(R(0.08032) * (u - E_K1 + R(5.476))) +
+                     fexp(R(0.06175) * (u - E_K1 - R(594.31)))) /
+                    (R(1) + fexp(R(-0.5143) * (u - E_K1 + R(4.753))));
+  const R K_1x = alpha_K1 / (alpha_K1 + beta_K1);
+  const R G_K1 = gk1 * fsqrt(ko / R(5.4));
+  const R ik1 = G_K1 * K_1x * (u - E_K1);
+  // calc_ikp, calc_ib
+  const R K_p = R(1.0) / (R(1) + fexp((R(7.488) - u) / R(5.98)));
+  const R ikp = gkp * K_p * (u - E_K1);
+  const R ib = gb * (u + R(59.87));
+  const R ik1t = ik1 + ikp + ib;
+  return un - dt * (ina + isi + ik1t + ik);
+}
+
+// ---- TP06  tp06_2d.py:242-1103 -----------------------------------------------
+// state order (tp06_2d.py:1073 call): cai casr cass nai Ki m h j xr1 xr2 xs r
+// s d f f2 fcass rr oo ; the first five are double in both precisions.
+template <typename R>
+__device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
+                                        int64_t c, R dt, const Params& P) {
+  typedef double D;
+  const R ko = (R)P.v[0], cao = (R)P.v[1], nao = (R)P.v[2];
+  const D Vc = P.v[3], Vsr = P.v[4], Vss = P.v[5];
+  const D Bufsr = P.v[8], Kbufsr = P.v[9], Bufss = P.v[10], Kbufss = P.v[11],
+          Vmaxup = P.v[12], Kup = P.v[13], Vrel = P.v[14], k1_ = P.v[15],
+          k2_ = P.v[16], k3 = P.v[17], k4 = P.v[18], EC = P.v[19],
+          maxsr = P.v[20], minsr = P.v[21], Vleak = P.v[22], Vxfer = P.v[23];
+  const R R_ = (R)P.v[24], F = (R)P.v[25], T = (R)P.v[26], RTONF = (R)P.v[27];
+  const D CAPACITANCE = P.v[28], Fd = P.v[25];
+  const R gkr = (R)P.v[29], pKNa = (R)P.v[30], gk1 = (R)P.v[31],
+          gna = (R)P.v[32], gbna = (R)P.v[33], KmK = (R)P.v[34],
+          KmNa = (R)P.v[35], knak = (R)P.v[36], gcal = (R)P.v[37],
+          gbca = (R)P.v[38], knaca = (R)P.v[39], KmNai = (R)P.v[40],
+          KmCa = (R)P.v[41], ksat = (R)P.v[42], n_ = (R)P.v[43],
+          gpca = (R)P.v[44], KpCa = (R)P.v[45], gpk = (R)P.v[46],
+          gto = (R)P.v[47], gks = (R)P.v[48];
+  const D inverseVcF = 1.0 / (Vc * Fd);
+  const D inversevssF2 = 1.0 / (2 * Vss * Fd);
+
+  D& cair = ref<D>(S, 0, c);
+  D& casrr = ref<D>(S, 1, c);
+  D& cassr = ref<D>(S, 2, c);
+  D& nair = ref<D>(S, 3, c);
+  D& Kir = ref<D>(S, 4, c);
+  const D caiD = cair, casrD = casrr, cassD = cassr, naiD = nair, KiD = Kir;
+  const R cai = (R)caiD, cass = (R)cassD, nai = (R)naiD, Ki = (R)KiD;
+
+  const R Ek = RTONF * flog(ko / Ki);
+  const R Ena = RTONF * flog(nao / nai);
+  const R Eks = RTONF * flog((ko + pKNa * nao) / (Ki + pKNa * nai));
+  const R Eca = R(0.5) * RTONF * flog(cao / cai);
+
+  // calc_ina
+  R ina;
+  {
+    R& mr = ref<R>(S, 5, c);
+    R& hr = ref<R>(S, 6, c);
+    R& jr = ref<R>(S, 7, c);
+    const R alpha_m = R(1.0) / (R(1.0) + fexp((R(-60.0) - u) / R(5.0)));
+    const R beta_m = R(0.1) / (R(1.0) + fexp((u + R(35.0)) / R(5.0))) +
+                     R(0.1) / (R(1.0) + fexp((u - R(50.0)) / R(200.0)));
+    const R tau_m = alpha_m * beta_m;
+    const R m_inf = R(1.0) / ((R(1.0) + fexp((R(-56.86) - u) / R(9.03))) *
+                              (R(1.0) + fexp((R(-56.86) - u) / R(9.03))));
+    R alpha_h, beta_h;
+    if (u >= R(-40.0)) {
+      alpha_h = R(0.0);
+      beta_h = R(0.77) / (R(0.13) * (R(1.0) + fexp(-(u + R(10.66)) / R(11.1))));
+    } else {
+      alpha_h = R(0.057) * fexp(-(u + R(80.0)) / R(6.8));
+      beta_h = R(2.7) * fexp(R(0.079) * u) + R(310000.0) * fexp(R(0.3485) * u);
+    }
+    const R tau_h = R(1.0) / (alpha_h + beta_h);
+    const R h_inf = R(1.0) / ((R(1.0) + fexp((u + R(71.55)) / R(7.43))) *
+                              (R(1.0) + fexp((u + R(71.55)) / R(7.43))));
+    R alpha_j, beta_j;
+    if (u >= R(-40.0)) {
+      alpha_j = R(0.0);
+      beta_j = R(0.6) * fexp(R(0.057) * u) /
+               (R(1.0) + fexp(R(-0.1) * (u + R(32.0))));
+    } else {
+      alpha_j = (R(-25428.0) * fexp(R(0.2444) * u) -
+                 R(6.948e-06) * fexp(R(-0.04391) * u)) *
+                (u + R(37.78)) / (R(1.0) + fexp(R(0.311) * (u + R(79.23))));
+      beta_j = R(0.02424) * fexp(R(-0.01052) * u) /
+               (R(1.0) + fexp(R(-0.1378) * (u + R(40.14))));
+    }
+    const R tau_j = R(1.0) / (alpha_j + beta_j);
+    const R j_inf = h_inf;
+    const R m = m_inf - (m_inf - mr) * fexp(-dt / tau_m);
+    const R h = h_inf - (h_inf - hr) * fexp(-dt / tau_h);
+    const R j = j_inf - (j_inf - jr) * fexp(-dt / tau_j);
+    mr = m;
+    hr = h;
+    jr = j;
+    ina = gna * m * m * m * h * j * (u - Ena);
+  }
+  // calc_ical
+  R ical;
+  {
+    R& dr = ref<R>(S, 13, c);
+    R& fr = ref<R>(S, 14, c);
+    R& f2r = ref<R>(S, 15, c);
+    R& fcr = ref<R>(S, 16, c);
+    const R d_inf = R(1.0) / (R(1.0) + fexp((R(-8) - u) / R(7.5)));
+    const R Ad = R(1.4) / (R(1.0) + fexp((R(-35) - u) / R(13))) + R(0.25);
+    const R Bd = R(1.4) / (R(1.0) + fexp((u + R(5)) / R(5)));
+    const R Cd = R(1.0) / (R(1.0) + fexp((R(50) - u) / R(20)));
+    const R tau_d = Ad * Bd + Cd;
+    const R f_inf = R(1.0) / (R(1.0) + fexp((u + R(20)) / R(7)));
+    const R Af = R(1102.5) * fexp(-(u + R(27)) * (u + R(27)) / R(225));
+    const R Bf = R(200.0) / (R(1) + fexp((R(13) - u) / R(10.0)));
+    const R Cf = R(180.0) / (R(1) + fexp((u + R(30)) / R(10))) + R(20);
+    const R tau_f = Af + Bf + Cf;
+    const R f2_inf = R(0.67) / (R(1.0) + fexp((u + R(35)) / R(7))) + R(0.33);
+    const R Af2 = R(600) * fexp(-(u + R(25)) * (u + R(25)) / R(170));
+    const R Bf2 = R(31) / (R(1.0) + fexp((R(25) - u) / R(10)));
+    const R Cf2 = R(16) / (R(1.0) + fexp((u + R(30)) / R(10)));
+    const R tau_f2 = Af2 + Bf2 + Cf2;
+    const R fcass_inf = R(0.6) / (R(1) + cass / R(0.05) * (cass / R(0.05))) + R(0.4);
+    const R tau_fcass = R(80.0) / (R(1) + cass / R(0.05) * (cass / R(0.05))) + R(2.0);
+    const R d = d_inf - (d_inf - dr) * fexp(-dt / tau_d);
+    const R f = f_inf - (f_inf - fr) * fexp(-dt / tau_f);
+    const R f2 = f2_inf - (f2_inf - f2r) * fexp(-dt / tau_f2);
+    const R fcass = fcass_inf - (fcass_inf - fcr) * fexp(-dt / tau_fcass);
+    dr = d;
+    fr = f;
+    f2r = f2;
+    fcr = fcass;
+    ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * (F * F / (R_ * T)) *
+           (R(0.25) * fexp(R(2) * (u - R(15)) * F / (R_ * T)) * cass - cao) /
+           (fexp(R(2) * (u - R(15)) * F / (R_ * T)) - R(1.0));
+  }
+  // calc_ito
+  R ito;
+  {
+    R& rr_ = ref<R>(S, 11, c);
+    R& sr = ref<R>(S, 12, c);
+    const R r_inf = R(1.0) / (R(1.0) + fexp((R(20) - u) / R(6.0)));
+    const R s_inf = R(1.0) / (R(1.0) + fexp((u + R(20)) / R(5.0)));
+    const R tau_r = R(9.5) * fexp(-(u + R(40.0)) * (u + R(40.0)) / R(1800.0)) + R(0.8);
+    const R tau_s = R(85.0) * fexp(-(u + R(45.0)) * (u + R(45.0)) / R(320.0)) +
+                    R(5.0) / (R(1.0) + fexp((u - R(20.0)) / R(5.0))) + R(3.0);
+    const R s = s_inf - (s_inf - sr) * fexp(-dt / tau_s);
+    const R r = r_inf - (r_inf - rr_) * fexp(-dt / tau_r);
+    sr = s;
+    rr_ = r;
+    ito = gto * r * s * (u - Ek);
+  }
+  // calc_ikr
+  R ikr;
+  {
+    R& x1r = ref<R>(S, 8, c);
+    R& x2r = ref<R>(S, 9, c);
+    const R xr1_inf = R(1.0) / (R(1.0) + fexp((R(-26.0) - u) / R(7.0)));
+    const R axr1 = R(450.0) / (R(1.0) + fexp((R(-45.0) - u) / R(10.0)));
+    const R bxr1 = R(6.0) / (R(1.0) + fexp((u - R(-30.0)) / R(11.5)));
+    const R tau_xr1 = axr1 * bxr1;
+    const R xr2_inf = R(1.0) / (R(1.0) + fexp((u - R(-88.0)) / R(24.0)));
+    const R axr2 = R(3.0) / (R(1.0) + fexp((R(-60.0) - u) / R(20.0)));
+    const R bxr2 = R(1.12) / (R(1.0) + fexp((u - R(60.0)) / R(20.0)));
+    const R tau_xr2 = axr2 * bxr2;
+    const R xr1 = xr1_inf - (xr1_inf - x1r) * fexp(-dt / tau_xr1);
+    const R xr2 = xr2_inf - (xr2_inf - x2r) * fexp(-dt / tau_xr2);
+    x1r = xr1;
+    x2r = xr2;
+    ikr = gkr * fsqrt(ko / R(5.4)) * xr1 * xr2 * (u - Ek);
+  }
+  // calc_iks
+  R iks;
+  {
+    R& xsr = ref<R>(S, 10, c);
+    const R Axs = R(1400.0) / fsqrt(R(1.0) + fexp((R(5.0) - u) / R(6)));
+    const R Bxs = R(1.0) / (R(1.0) + fexp((u - R(35.0)) / R(15.0)));
+    const R tau_xs = Axs * Bxs + R(80);
+    const R xs_inf = R(1.0) / (R(1.0) + fexp((R(-5.0) - u) / R(14.0)));
+    const R xs = xs_inf - (xs_inf - xsr) * fexp(-dt / tau_xs);
+    xsr = xs;
+    iks = gks * xs * xs * (u - Eks);
+  }
+  // calc_ik1
+  const R ak1 = R(0.1) / (R(1.0) + fexp(R(0.06) * (u - Ek - R(200))));
+  const R bk1 = (R(3.0) * fexp(R(0.0002) * (u - Ek + R(100))) +
+                 fexp(R(0.1) * (u - Ek - R(10)))) /
+                (R(1.0) + fexp(R(-0.5) * (u - Ek)));
+  const R rec_iK1 = ak1 / (ak1 + bk1);
+  const R ik1 = gk1 * rec_iK1 * (u - Ek);
+  // calc_inaca
+  const R inaca =
+      knaca * (R(1.0) / (KmNai * KmNai * KmNai + nao * nao * nao)) *
+      (R(1.0) / (KmCa + cao)) *
+      (R(1.0) / (R(1) + ksat * fexp((n_ - R(1)) * u * F / (R_ * T)))) *
+      (fexp(n_ * u * F / (R_ * T)) * nai * nai * nai * cao -
+       fexp((n_ - R(1)) * u * F / (R_ * T)) * nao * nao * nao * cai * R(2.5));
+  // calc_inak
+  const R rec_iNaK = R(1.0) / (R(1.0) + R(0.1245) * fexp(R(-0.1) * u * F / (R_ * T)) +
+                               R(0.0353) * fexp(-u * F / (R_ * T)));
+  const R inak = knak * (ko / (ko + KmK)) * (nai / (nai + KmNa)) * rec_iNaK;
+  const R ipca = gpca * cai / (KpCa + cai);
+  const R rec_ipK = R(1.0) / (R(1.0) + fexp((R(25) - u) / R(5.98)));
+  const R ipk = gpk * rec_ipK * (u - Ek);
+  const R ibna = gbna * (u - Ena);
+  const R ibca = gbca * (u - Eca);
+  // calc_irel (double: feeds the SR/SS calcium balance)
+  D irel;
+  {
+    R& rrr = ref<R>(S, 17, c);
+    R& oor = ref<R>(S, 18, c);
+    const D kCaSR = maxsr - (maxsr - minsr) / (1 + EC / casrD * (EC / casrD));
+    const D k1 = k1_ / kCaSR;
+    const D k2 = k2_ * kCaSR;
+    D rr = (D)rrr;
+    const D drr = k4 * (1 - rr) - k2 * cassD * rr;
+    rr += (D)dt * drr;
+    const D oo = k1 * cassD * cassD * rr / (k3 + k1 * cassD * cassD);
+    rrr = (R)rr;
+    oor = (R)oo;
+    irel = Vrel * oo * (casrD - cassD);
+  }
+  const D ileak = Vleak * (casrD - caiD);
+  const D iup = Vmaxup / (1.0 + Kup * Kup / (caiD * caiD));
+  const D ixfer = Vxfer * (cassD - caiD);
+  const D dtD = (D)dt;
+  // calc_casr: closed-form buffer quadratic
+  {
+    const D CaCSQN = Bufsr * casrD / (casrD + Kbufsr);
+    const D dCaSR = dtD * (iup - irel - ileak);
+    const D bjsr = Bufsr - CaCSQN - dCaSR - casrD + Kbufsr;
+    const D cjsr = Kbufsr * (CaCSQN + dCaSR + casrD);
+    casrr = (sqrt(bjsr * bjsr + 4 * cjsr) - bjsr) / 2;
+  }
+  // calc_cass
+  {
+    const D CaSSBuf = Bufss * cassD / (cassD + Kbufss);
+    const D dCaSS = dtD * (-ixfer * (Vc / Vss) + irel * (Vsr / Vss) +
+                           -(D)ical * inversevssF2 * CAPACITANCE);
+    const D bcss = Bufss - CaSSBuf - dCaSS - cassD + Kbufss;
+    const D ccss = Kbufss * (CaSSBuf + dCaSS + cassD);
+    cassr = (sqrt(bcss * bcss + 4 * ccss) - bcss) / 2;
+  }
+  // calc_cai's result is discarded by the tuple store (tp06_2d.py:1098).
+  // calc_nai / calc_ki
+  {
+    const D dNai = -((D)ina + (D)ibna + 3 * (D)inak + 3 * (D)inaca) * inverseVcF *
+                   CAPACITANCE;
+    nair = naiD + dtD * dNai;
+    const D dKi = -((D)ik1 + (D)ito + (D)ikr + (D)iks - 2 * (D)inak + (D)ipk) *
+                  inverseVcF * CAPACITANCE;
+    Kir = KiD + dtD * dKi;
+  }
+  return un - dt * (ikr + iks + ik1 + ito + ina + ibna + ical + ibca + inak +
+                    inaca + ipca + ipk);
+}
+
+// ---- Courtemanche  courtemanche_2d.py:180-641 -------------------------------
+// state order = the kernel's parameter order (courtemanche_2d.py:568-569):
+// nai ki cai caup carel m h j_ d f oa oi ua ui xs xr fca irel vrel urel wrel;
+// the model passes its xr array in the xs slot and vice versa.
+template <typename R>
+__device__ __forceinline__ R crn_gate(R x, R x_inf, R tau_x, R dt) {
+  return x_inf - (x_inf - x) * fexp(-dt / tau_x);
+}
+
+template <typename R>
+__device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
+                                       int64_t c, R dt, const Params& P) {
+  const R gna = (R)P.v[0], gnab = (R)P.v[1], gk1 = (R)P.v[2], gks = (R)P.v[4],
+          gto = (R)P.v[5], gcal = (R)P.v[6], gcab = (R)P.v[7],
+          gkur_coeff = (R)P.v[8], F = (R)P.v[9], T = (R)P.v[10],
+          R_ = (R)P.v[11], Vj = (R)P.v[13], Vup = (R)P.v[14],
+          Vrel = (R)P.v[15], ibk = (R)P.v[16], cao = (R)P.v[17],
+          nao = (R)P.v[18], ko = (R)P.v[19], caupmax = (R)P.v[20],
+          kup = (R)P.v[21], kmnai = (R)P.v[22], kmko = (R)P.v[23],
+          kmnancx = (R)P.v[24], kmcancx = (R)P.v[25], ksatncx = (R)P.v[26],
+          kmcmdn = (R)P.v[27], kmtrpn = (R)P.v[28], kmcsqn = (R)P.v[29],
+          trpnmax = (R)P.v[30], cmdnmax = (R)P.v[31], csqnmax = (R)P.v[32],
+          inacamax = (R)P.v[33], inakmax = (R)P.v[34], ipcamax = (R)P.v[35],
+          krel = (R)P.v[36], iupmax = (R)P.v[37], kq10 = (R)P.v[38];
+  R& nair = ref<R>(S, 0, c);
+  R& kir = ref<R>(S, 1, c);
+  R& cair = ref<R>(S, 2, c);
+  R& caupr = ref<R>(S, 3, c);
+  R& carelr = ref<R>(S, 4, c);
+  const R nai = nair, ki = kir, cai = cair, caup = caupr, carel = carelr;
+  // calc_equilibrum_potentials
+  const R ena = R_ * T / F * flog(nao / nai);
+  const R ek = R_ * T / F * flog(ko / ki);
+  const R eca = R_ * T / (R(2) * F) * flog(cao / cai);
+  // calc_gating_m / h / j
+  R m, h, j;
+  {
+    R& mr = ref<R>(S, 5, c);
+    R am;
+    if (u == R(-47.13)) {
+      am = R(3.2);
+    } else {
+      am = R(0.32) * (u + R(47.13)) / (R(1) - fexp(R(-0.1) * (u + R(47.13))));
+    }
+    const R bm = R(0.08) * fexp(-u / R(11));
+    const R m_inf = am / (am + bm);
+    const R tau_m = R(1) / (am + bm);
+    m = crn_gate(mr, m_inf, tau_m, dt);
+    mr = m;
+  }
+  {
+    R& hr = ref<R>(S, 6, c);
+    R ah, bh;
+    if (u >= R(-40)) {
+      ah = R(0);
+      bh = R(1) / (R(0.13) * (R(1) + fexp(-(u + R(10.66)) / R(11.1))));
+    } else {
+      ah = R(0.135) * fexp(-(R(80) + u) / R(6.8));
+      bh = R(3.56) * fexp(R(0.079) * u) + R(310000) * fexp(R(0.35) * u);
+    }
+    const R h_inf = ah / (ah + bh);
+    const R tau_h = R(1) / (ah + bh);
+    h = crn_gate(hr, h_inf, tau_h, dt);
+    hr = h;
+  }
+  {
+    R& jr = ref<R>(S, 7, c);
+    R aj, bj;
+    if (u >= R(-40)) {
+      aj = R(0);
+      bj = R(0.3) * fexp(R(-2.535e-07) * u) / (R(1) + fexp(R(-0.1) * (u + R(32))));
+    } else {
+      aj = (R(-127140) * fexp(R(0.2444) * u) - R(3.474e-05) * fexp(R(-0.04391) * u)) *
+           (u + R(37.78)) / (R(1) + fexp(R(0.311) * (u + R(79.23))));
+      bj = R(0.1212) * fexp(R(-0.01052) * u) / (R(1) + fexp(R(-0.1378) * (u + R(40.14))));
+    }
+    const R j_inf = aj / (aj + bj);
+    const R tau_j = R(1) / (aj + bj);
+    j = j_inf - (j_inf - jr) * fexp(-dt / tau_j);
+    jr = j;
+  }
+  const R ina = gna * (m * (m * m)) * h * j * (u - ena);
+  const R ik1 = gk1 * (u - ek) / (R(1) + fexp(R(0.07) * (u + R(80))));
+  // calc_ito
+  R ito;
+  {
+    R& oar = ref<R>(S, 10, c);
+    R& oir = ref<R>(S, 11, c);
+    const R ao = R(0.65) / (fexp(-(u + R(10)) / R(8.5)) + fexp(-(u - R(30)) / R(59.0)));
+    const R bo = R(0.65) / (R(2.5) + fexp((u + R(82)) / R(17.0)));
+    const R tau_o = R(1) / (kq10 * (ao + bo));
+    const R o_inf = R(1) / (R(1) + fexp(-(u + R(20.47)) / R(17.54)));
+    const R aoi = R(1) / (R(18.53) + fexp((u + R(113.7)) / R(10.95)));
+    const R boi = R(1) / (R(35.56) + fexp(-(u + R(1.26)) / R(7.44)));
+    const R tau_oi = R(1) / (kq10 * (aoi + boi));
+    const R oi_inf = R(1) / (R(1) + fexp((u + R(43.1)) / R(5.3)));
+    const R oa = crn_gate(oar, o_inf, tau_o, dt);
+    const R oi = crn_gate(oir, oi_inf, tau_oi, dt);
+    oar = oa;
+    oir = oi;
+    ito = gto * (oa * (oa * oa)) * oi * (u - ek);
+  }
+  // calc_ikur
+  R ikur;
+  {
+    R& uar = ref<R>(S, 12, c);
+    R& uir = ref<R>(S, 13, c);
+    const R gkur = R(0.005) + R(0.05) / (R(1) + fexp(-(u - R(15)) / R(13.0)));
+    const R aua = R(0.65) / (fexp(-(u + R(10)) / R(8.5)) + fexp(-(u - R(30)) / R(59.0)));
+    const R bua = R(0.65) / (R(2.5) + fexp((u + R(82)) / R(17.0)));
+    const R tau_ua = R(1) / (kq10 * (aua + bua));
+    const R ua_inf = R(1) / (R(1) + fexp(-(u + R(30.3)) / R(9.6)));
+    const R aui = R(1) / (R(21) + fexp(-(u - R(185)) / R(28.0)));
+    const R bui = fexp((u - R(158)) / R(16.0));
+    const R tau_ui = R(1) / (kq10 * (aui + bui));
+    const R ui_inf = R(1) / (R(1) + fexp((u - R(99.45)) / R(27.48)));
+    const R ua = crn_gate(uar, ua_inf, tau_ua, dt);
+    const R ui = crn_gate(uir, ui_inf, tau_ui, dt);
+    uar = ua;
+    uir = ui;
+    ikur = gkur_coeff * gkur * (ua * (ua * ua)) * ui * (u - ek);
+  }
+  // calc_ikr on the kernel's xr slot (index 15), gkr hard-coded to 0.0294
+  R ikr;
+  {
+    R& xrr = ref<R>(S, 15, c);
+    const R gkr = R(0.0294);
+    const R axr = R(0.0003) * (u + R(14.1)) / (R(1) - fexp(-(u + R(14.1)) / R(5)));
+    const R bxr = R(7.3898e-05) * (u - R(3.3328)) / (fexp((u - R(3.3328)) / R(5.1237)) - R(1));
+    const R tau_xr = R(1) / (axr + bxr);
+    const R xr_inf = R(1) / (R(1) + fexp(-(u + R(14.1)) / R(6.5)));
+    const R xr = crn_gate(xrr, xr_inf, tau_xr, dt);
+    xrr = xr;
+    ikr = gkr * xr * (u - ek) / (R(1) + fexp((u + R(15)) / R(22.4)));
+  }
+  // calc_iks on the kernel's xs slot (index 14)
+  R iks;
+  {
+    R& xsr = ref<R>(S, 14, c);
+    const R axs = R(4e-05) * (u - R(19.9)) / (R(1) - fexp(-(u - R(19.9)) / R(17)));
+    const R bxs = R(3.5e-05) * (u - R(19.9)) / (fexp((u - R(19.9)) / R(9)) - R(1));
+    const R tau_xs = R(1) / (R(2) * (axs + bxs));
+    const R xs_inf = R(1) / fsqrt(R(1) + fexp(-(u - R(19.9)) / R(12.7)));
+    const R xs = crn_gate(xsr, xs_inf, tau_xs, dt);
+    xsr = xs;
+    iks = gks * (xs * xs) * (u - ek);
+  }
+  // calc_ical
+  R ical;
+  {
+    R& dr = ref<R>(S, 8, c);
+    R& fr = ref<R>(S, 9, c);
+    R& fcar = ref<R>(S, 16, c);
+    const R tau_d = (R(1) - fexp(-(u + R(10)) / R(6.24))) /
+                    (R(0.035) * (u + R(10)) * (R(1) + fexp(-(u + R(10)) / R(6.24))));
+    const R d_inf = R(1) / (R(1) + fexp(-(u + R(10)) / R(8.0)));
+    const R tau_f = R(9) / (R(0.0197) * fexp(-R(0.0337 * 0.0337) * ((u + R(10)) * (u + R(10)))) +
+                            R(0.02));
+    const R f_inf = R(1) / (R(1) + fexp((u + R(28)) / R(6.9)));
+    const R tau_fca = R(2);
+    const R fca_inf = R(1) / (R(1) + cai / R(0.00035));
+    const R d = crn_gate(dr, d_inf, tau_d, dt);
+    const R f = crn_gate(fr, f_inf, tau_f, dt);
+    const R fca = crn_gate(fcar, fca_inf, tau_fca, dt);
+    dr = d;
+    fr = f;
+    fcar = fca;
+    ical = gcal * d * f * fca * (u - R(65));
+  }
+  // calc_inak
+  const R sig = R(1 / 7.0) * (fexp(nao / R(67.3)) - R(1));
+  const R fnak = R(1) / (R(1) + R(0.1245) * fexp(R(-0.1) * (F * u) / (R_ * T)) +
+                         R(0.0365) * sig * fexp(-(F * u) / (R_ * T)));
+  const R inak = inakmax * fnak * (R(1) / (R(1) + fpow(kmnai / nai, R(1.5)))) *
+                 (ko / (ko + kmko));
+  // calc_inaca
+  const R gamma = R(0.35);
+  const R exp_term = fexp(gamma * (F * u) / (R_ * T));
+  const R exp_rev_term = fexp((gamma - R(1)) * (F * u) / (R_ * T));
+  const R nao3 = nao * (nao * nao);
+  const R numerator =
+      inacamax * (exp_term * (nai * (nai * nai)) * cao - exp_rev_term * nao3 * cai);
+  const R term1 = kmnancx * (kmnancx * kmnancx) + nao3;
+  const R term2 = kmcancx + cao;
+  const R term3 = R(1) + ksatncx * exp_rev_term;
+  const R inaca = numerator / (term1 * term2 * term3);
+  const R ibca = gcab * (u - eca);
+  const R ibna = gnab * (u - ena);
+  const R ipca = ipcamax * cai / (cai + R(0.0005));
+  // calc_irel (uses the irel stored by the previous step)
+  R irel;
+  {
+    R& irr = ref<R>(S, 17, c);
+    R& vrr = ref<R>(S, 18, c);
+    R& urr = ref<R>(S, 19, c);
+    R& wrr = ref<R>(S, 20, c);
+    const R tau_u = R(8);
+    const R Fn = R(1e-12) * Vrel * irr - R(5 * 1e-13) / F * (R(0.5) * ical - R(0.2) * inaca);
+    const R u_inf = R(1) / (R(1) + fexp(-(Fn - R(3.4175e-13)) / R(1.367e-15)));
+    const R tau_v = R(1.91) + R(2.09) / (R(1) + fexp(-(Fn - R(3.4175e-13)) / R(1.367e-15)));
+    const R v_inf = R(1) - R(1) / (R(1) + fexp(-(Fn - R(6.835e-14)) / R(1.367e-15)));
+    const R tau_w = R(6) * (R(1) - fexp(-(u - R(7.9)) / R(5.0))) /
+                    ((R(1) + R(0.3) * fexp(-(u - R(7.9)) / R(5.0))) * (u - R(7.9)));
+    const R w_inf = R(1) - R(1) / (R(1) + fexp(-(u - R(40)) / R(17.0)));
+    const R urel = crn_gate(urr, u_inf, tau_u, dt);
+    const R vrel = crn_gate(vrr, v_inf, tau_v, dt);
+    const R wrel = crn_gate(wrr, w_inf, tau_w, dt);
+    urr = urel;
+    vrr = vrel;
+    wrr = wrel;
+    irel = krel * (urel * urel) * vrel * wrel * (carel - cai);
+    irr = irel;
+  }
+  const R itr = (caup - carel) / R(180);
+  const R iup = iupmax / (R(1) + kup / cai);
+  const R iupleak = caup / caupmax * iupmax;
+  caupr = caup + dt * (iup - iupleak - itr * (Vrel / Vup));
+  nair = nai + dt * ((R(-3) * inak - R(3) * inaca - ibna - ina) / (F * Vj));
+  kir = ki + dt * ((R(2) * inak - ik1 - ito - ikur - ikr - iks - ibk) / (F * Vj));
+  {
+    const R B1 = (R(2) * inaca - ipca - ical - ibca) / (R(2) * F * Vj) +
+                 (Vup * (iupleak - iup) + irel * Vrel) / Vj;
+    const R B2 = R(1) + trpnmax * kmtrpn / ((cai + kmtrpn) * (cai + kmtrpn)) +
+                 cmdnmax * kmcmdn / ((cai + kmcmdn) * (cai + kmcmdn));
+    cair = cai + dt * (B1 / B2);
+  }
+  carelr = carel + dt * ((itr - irel) /
+                         (R(1) + csqnmax * kmcsqn / ((carel + kmcsqn) * (carel + kmcsqn))));
+  return un - dt * (ina + ik1 + ito + ikur + ikr + iks + ical + ipca + inak +
+                    inaca + ibna + ibca);
+}
+
+template <int MODEL, typename R>
+__device__ __forceinline__ R ionic_update(R u, R un, const StatePtrs& S,
+                                          int64_t c, R dt, const Params& P) {
+  if constexpr (MODEL == 0) return ionic_ap<R>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 1) return ionic_barkley<R>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 2) return ionic_ms<R>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 3) return ionic_fk<R>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 4) return ionic_bo<R>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 5) return ionic_lr91<R>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 6) return ionic_tp06<R>(u, un, S, c, dt, P);
+  else return ionic_crn<R>(u, un, S, c, dt, P);
+}
+
+// number of state arrays per model (state_vars minus u)
+__host__ __device__ constexpr int model_n_state(int model) {
+  return model == 0 ? 1 : model == 1 ? 1 : model == 2 ? 1 : model == 3 ? 2
+       : model == 4 ? 3 : model == 5 ? 7 : model == 6 ? 19 : 21;
+}
+__host__ __device__ constexpr int model_n_params(int model) {
+  return model == 0 ? 5 : model == 1 ? 3 : model == 2 ? 5 : model == 3 ? 11
+       : model == 4 ? 28 : model == 5 ? 15 : model == 6 ? 49 : 39;
+}
+// FP64 storage of state k under precision prec (1 = FP32 path)
+__host__ __device__ constexpr bool state_is_f64(int model, int prec, int k) {
+  return prec == 0 ? true : (model == 6 && k < 5);
+}
+
+}  // namespace fw

@@ -1,0 +1,92 @@
+// fw_step_inst.hip -- instantiates the fused step kernel for one
+// (precision, model) pair, chosen at compile time by FW_PREC and FW_MODEL.
+// The build compiles this file once per pair (FP64 builds with
+// -ffp-contract=off, FP32 builds with contraction allowed).
+#include <type_traits>
+
+#include "fw_internal.hpp"
+#include "fw_kernels.hpp"
+
+#ifndef FW_PREC
+#error "FW_PREC must be defined"
+#endif
+#ifndef FW_MODEL
+#error "FW_MODEL must be defined"
+#endif
+
+#define FW_CAT4_(a, b, c, d) a##b##c##d
+#define FW_CAT4(a, b, c, d) FW_CAT4_(a, b, c, d)
+#define FW_LAUNCHER FW_CAT4(fw_step_launch_p, FW_PREC, _m, FW_MODEL)
+
+namespace fw {
+
+using Real = std::conditional_t<FW_PREC == 0, double, float>;
+
+template <int NDIM, int ST>
+static void launch(const fw_plan* plan, const KArgs<Real>& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC>), dim3(plan->grid),
+                     dim3(kBlock), 0, s, a);
+}
+
+int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
+                const StatePtrs& st, const TrackArgs& tr, hipStream_t s) {
+  const fw_plan_desc& d = plan->desc;
+  KArgs<Real> a;
+  a.u = static_cast<const Real*>(u);
+  a.u_new = static_cast<Real*>(u_new);
+  a.st = st;
+  a.code = d.code;
+  a.ni = d.shape[0];
+  a.nj = d.shape[1];
+  a.nk = d.ndim == 3 ? d.shape[2] : 1;
+  a.N = plan->N;
+  a.nblocks = plan->nblocks;
+  a.per_xcd = plan->per_xcd;
+  for (int q = 0; q < 9; ++q) {
+    if (q < d.n_fields) {
+      a.f[q].mode = d.fields[q].mode;
+      a.f[q].axis = d.fields[q].axis;
+      a.f[q].value = d.fields[q].value;
+      a.f[q].data = d.fields[q].data;
+    } else {
+      a.f[q] = Field{0, 0, 0.0, nullptr};
+    }
+  }
+  if (d.stencil == FW_STENCIL_WEIGHTS) a.f[0].mode = plan->K;
+  a.w = d.weights;
+  a.D = d.D_model;
+  a.dtd = d.dt;
+  a.dr2 = d.dr2;
+  a.dt = (Real)d.dt;
+  a.s = (Real)(d.D_model * d.dt / d.dr2);
+  for (int q = 0; q < 64; ++q) a.P.v[q] = q < d.n_params ? d.params[q] : 0.0;
+  a.act = tr.act;
+  a.act_thr = (Real)tr.act_thr;
+  a.act_on = tr.act_on && tr.act != nullptr;
+  a.step = tr.step;
+  a.step_in = tr.step_in;
+  a.step_out = tr.step_out;
+  a.n_probe = tr.n_probe;
+  a.probe_idx = tr.probe_idx;
+  a.probe_var = tr.probe_var;
+  a.probe_buf = tr.probe_buf;
+  a.probe_row = tr.probe_buf ? tr.probe_row : -1;
+  const int st_kind = d.stencil;
+  if (d.ndim == 2) {
+    if (st_kind == 0) launch<2, 0>(plan, a, s);
+    else if (st_kind == 1) launch<2, 1>(plan, a, s);
+    else launch<2, 2>(plan, a, s);
+  } else {
+    if (st_kind == 0) launch<3, 0>(plan, a, s);
+    else if (st_kind == 1) launch<3, 1>(plan, a, s);
+    else launch<3, 2>(plan, a, s);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("step launch failed: %s", hipGetErrorString(e));
+    return FW_E_HIP;
+  }
+  return FW_OK;
+}
+
+}  // namespace fw

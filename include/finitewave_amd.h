@@ -1,0 +1,209 @@
+/*
+ * finitewave_amd.h -- C ABI of the MI355X (gfx950) stepping engine.
+ *
+ * Plain C: pointers, sizes, status codes.  Every array argument is a device
+ * pointer (HIP), C-order over the grid shape, unless stated otherwise.  No
+ * entry point allocates device memory except fw_plan_create (a small
+ * parameter block).  `stream` is a hipStream_t passed as void* (NULL = the
+ * default stream).  All launches are asynchronous on that stream.
+ *
+ * Status: 0 on success, negative on error; fw_last_error() returns a
+ * thread-local message for the last failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the finitewave source):
+ *   fw_compute_weights  <- Stencil.compute_weights's njit loops
+ *                          cpuwave2D/stencil/isotropic_stencil_2d.py:161-204,
+ *                          cpuwave2D/stencil/asymmetric_stencil_2d.py:291-433,
+ *                          cpuwave2D/stencil/symmetric_stencil_2d.py:159-250,
+ *                          cpuwave3D/stencil/isotropic_stencil_3d.py:112-149,
+ *                          cpuwave3D/stencil/asymmetric_stencil_3d.py:163-458
+ *   fw_diffusion        <- diffusion_kernel_{2d,3d}_{iso,aniso}(u_new, u, w,
+ *                          indexes): isotropic_stencil_2d.py:97-132,
+ *                          asymmetric_stencil_2d.py:164-201,
+ *                          isotropic_stencil_3d.py:77-110,
+ *                          asymmetric_stencil_3d.py:108-160
+ *   fw_ionic            <- ionic_kernel_{2d,3d}(u_new, u, *state, indexes, dt,
+ *                          *params) of each model, e.g.
+ *                          cpuwave2D/model/aliev_panfilov_2d.py:173-203,
+ *                          cpuwave2D/model/tp06_2d.py:987-1103
+ *   fw_plan_create/fw_run
+ *                       <- the per-step body of CardiacModel.run()
+ *                          (core/model/cardiac_model.py:164-178): diffusion +
+ *                          ionic + activation-time tracker fused into one
+ *                          kernel per step, weights rebuilt from the mesh.
+ *   fw_stim_apply       <- Stim*.stimulate (cpuwave2D/stimulation/*.py,
+ *                          cpuwave3D/stimulation/*.py)
+ *   fw_act_time_update  <- ActivationTime2DTracker._track
+ *                          (cpuwave2D/tracker/activation_time_2d_tracker.py:61-63)
+ */
+#ifndef FINITEWAVE_AMD_H
+#define FINITEWAVE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FW_ABI_VERSION 1
+
+/* status codes */
+#define FW_OK 0
+#define FW_E_INVALID (-1)     /* bad argument */
+#define FW_E_UNSUPPORTED (-2) /* combination not built */
+#define FW_E_HIP (-3)         /* HIP runtime error */
+#define FW_E_NOMEM (-4)       /* allocation failed */
+
+/* ionic models (ids in the order of the reference's model packages) */
+#define FW_MODEL_ALIEV_PANFILOV 0
+#define FW_MODEL_BARKLEY 1
+#define FW_MODEL_MITCHELL_SCHAEFFER 2
+#define FW_MODEL_FENTON_KARMA 3
+#define FW_MODEL_BUENO_OROVIO 4
+#define FW_MODEL_LUO_RUDY91 5
+#define FW_MODEL_TP06 6
+#define FW_MODEL_COURTEMANCHE 7
+#define FW_N_MODELS 8
+
+/* stencils */
+#define FW_STENCIL_ISO 0     /* 5-point (2D) / 7-point (3D), weights rebuilt */
+#define FW_STENCIL_ANISO 1   /* 9-point (2D) / 19-point (3D), weights rebuilt */
+#define FW_STENCIL_WEIGHTS 2 /* precomputed (*shape, K) float64 weights */
+#define FW_STENCIL_SYM 3     /* fw_compute_weights only: symmetric 9-point */
+
+/* precision */
+#define FW_PREC_FP64 0 /* all state FP64; bit-for-bit the reference order */
+#define FW_PREC_FP32 1 /* FP32 state; TP06 cai/casr/cass/nai/Ki kept FP64 */
+
+/* half-step diffusion field storage */
+#define FW_FIELD_UNIFORM 0 /* one value */
+#define FW_FIELD_AXIS 1    /* table over one grid axis (data[shape[axis]]) */
+#define FW_FIELD_FULL 2    /* one value per node (data[N]) */
+
+/* node code bits (uint8 per node) */
+#define FW_CODE_CONDUCT 1 /* mesh == 1: takes part in the stencil */
+#define FW_CODE_FROZEN 2  /* special boundary: conducts, never updated */
+#define FW_CODE_PROBE 4   /* sampled by a point-probe tracker */
+
+#define FW_MAX_STATE 24
+#define FW_MAX_PARAMS 64
+#define FW_MAX_FIELDS 9
+
+typedef struct fw_field {
+  int32_t mode; /* FW_FIELD_* */
+  int32_t axis; /* for FW_FIELD_AXIS: 0, 1 or 2 */
+  double value; /* for FW_FIELD_UNIFORM */
+  const double* data; /* device, for FW_FIELD_AXIS / FW_FIELD_FULL */
+} fw_field;
+
+/* Description of one tissue simulation (one plan per simulation).
+ * fields: half-step diffusion coefficients, d[axis][component] at the face
+ * between node n and node n+1 along `axis` (the arrays the reference builds
+ * with np.roll): iso 2D {xx, yy}; iso 3D {xx, yy, zz};
+ * aniso 2D {xx, xy, yx, yy}; aniso 3D {xx, xy, xz, yx, yy, yz, zx, zy, zz}. */
+typedef struct fw_plan_desc {
+  int32_t ndim;      /* 2 or 3 */
+  int32_t model;     /* FW_MODEL_* */
+  int32_t stencil;   /* FW_STENCIL_ISO / ANISO / WEIGHTS */
+  int32_t precision; /* FW_PREC_* */
+  int64_t shape[3];  /* ni, nj, nk (nk = 1 for 2D) */
+  const uint8_t* code; /* device [N] */
+  int32_t n_fields;
+  fw_field fields[FW_MAX_FIELDS];
+  const double* weights; /* device [N*K], FW_STENCIL_WEIGHTS only */
+  double D_model, dt, dr2; /* dr2 = dr**2 as the host computed it */
+  int32_t n_params;
+  double params[FW_MAX_PARAMS]; /* reference ionic_kernel argument order */
+} fw_plan_desc;
+
+typedef struct fw_plan fw_plan;
+
+/* Per-run tracking.  act: int32 [N] step index of first crossing, -1 before
+ * (NULL: off).  act_flags: host [n_steps], nonzero = the tracker's gate is
+ * open on that step.  probes: probe_idx[n_probe] flat node index (device),
+ * probe_var[n_probe] (device) -1 = u before the step, k = state k after it;
+ * probe_rows: host [n_steps] row of probe_buf (double [rows*n_probe],
+ * device) to write on that step, -1 = none. */
+typedef struct fw_track_desc {
+  int32_t* act;
+  double act_thr;
+  const uint8_t* act_flags;
+  int32_t n_probe;
+  const int64_t* probe_idx;
+  const int32_t* probe_var;
+  double* probe_buf;
+  const int32_t* probe_rows;
+} fw_track_desc;
+
+typedef struct fw_stim_desc {
+  int32_t kind;       /* 0 = set voltage, 1 = add dt*current */
+  int64_t lo[3], hi[3]; /* box [lo, hi) clipped to the grid */
+  const uint8_t* mask; /* device [N] or NULL; nonzero = inside */
+  double value;       /* voltage, or dt*current already multiplied */
+  int32_t clamp;      /* 1: u = min(u, u_max) after adding */
+  double u_max;
+} fw_stim_desc;
+
+int fw_abi_version(void);
+const char* fw_last_error(void);
+
+/* Number of state arrays and parameters of a model, and whether state k is
+ * stored as FP64 under `precision` (1) or FP32 (0). */
+int fw_model_info(int32_t model, int32_t* n_state, int32_t* n_params);
+int fw_state_is_f64(int32_t model, int32_t precision, int32_t k);
+
+/* Weight builder: w[N*K] float64 raw weights (before the D*dt/dr**2 scale),
+ * kind FW_STENCIL_ISO / ANISO / SYM; m int8 [N] with 0/1 values;
+ * d: device pointers to the full half-step fields in the order above (sym:
+ * its four corner-averaged arrays xx, xy, yx, yy).  w must be zeroed. */
+int fw_compute_weights(int32_t ndim, int32_t kind, double* w, const int8_t* m,
+                       const double* const* d, const int64_t* shape,
+                       void* stream);
+
+/* One diffusion sweep over the active indexes with precomputed weights:
+ * u_new[c] = sum_k u[c + o_k] * w[c, k] (K = 5, 9, 7 or 19). */
+int fw_diffusion(int32_t ndim, int32_t K, double* u_new, const double* u,
+                 const double* w, const int64_t* indexes, int64_t n_indexes,
+                 const int64_t* shape, void* stream);
+
+/* One ionic sweep: state updated in place, u_new += dt * f(u, state)
+ * (u_new -= dt * I_ion for the physiological models).  state: host array of
+ * n_state device pointers in the reference kernel's argument order. */
+int fw_ionic(int32_t model, int32_t precision, void* u_new, const void* u,
+             void* const* state, int32_t n_state, const int64_t* indexes,
+             int64_t n_indexes, double dt, const double* params,
+             int32_t n_params, void* stream);
+
+/* Fused stepping: plan creation validates the description and copies it. */
+int fw_plan_create(const fw_plan_desc* desc, fw_plan** out);
+int fw_plan_destroy(fw_plan* plan);
+
+/* For FW_STENCIL_WEIGHTS plans: weights per node, 5 or 9 (2D), 7 or 19 (3D);
+ * the default is 9 / 19. */
+int fw_plan_set_weights_k(fw_plan* plan, int32_t K);
+
+/* n_steps fused steps starting at global step `step0`: step s reads `u`,
+ * writes `u_new`, then the two swap; after an odd n_steps the result is in
+ * the buffer passed as u_new.  state: host array of device pointers. */
+int fw_run(fw_plan* plan, void* u, void* u_new, void* const* state,
+           int64_t step0, int32_t n_steps, const fw_track_desc* track,
+           void* stream);
+
+/* Same step as fw_run with n_steps = 1, recorded into a HIP graph of
+ * n_steps nodes (even n_steps) and replayed; used for long stretches with
+ * no host events.  Falls back to plain launches when capture fails. */
+int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state,
+                 int64_t step0, int32_t n_steps, const fw_track_desc* track,
+                 void* stream);
+
+int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim,
+                  void* stream);
+
+/* One activation-time pass over all N nodes (act = step where u > thr first) */
+int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr,
+                       int32_t step, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -372,13 +372,19 @@ class OracleSim:
         fn(_p(self.u_new), _p(self.u), _p(self.weights), _p(self.idx, _I64),
            len(self.idx), *self.mesh.shape)
 
-    def _ionic(self) -> None:
-        L = lib()
+    def _ionic_args(self):
         names = [n for n, _ in self.spec["state"]]
-        ptrs = (_D * len(names))(*[_p(self.state[n]) for n in names])
-        params = np.array([float(self.params[n]) for n, _ in self.spec["params"]])
-        rc = L.orc_ionic(self.spec["id"], _p(self.u_new), _p(self.u), ptrs, _p(self.idx, _I64),
-                         len(self.idx), self.dt, _p(params))
+        key = tuple(self.state[n].ctypes.data for n in names)
+        if getattr(self, "_ion_key", None) != key:
+            self._ion_ptrs = (_D * len(names))(*[_p(self.state[n]) for n in names])
+            self._ion_params = np.array([float(self.params[n]) for n, _ in self.spec["params"]])
+            self._ion_key = key
+        return self._ion_ptrs, self._ion_params
+
+    def _ionic(self) -> None:
+        ptrs, params = self._ionic_args()
+        rc = lib().orc_ionic(self.spec["id"], _p(self.u_new), _p(self.u), ptrs,
+                             _p(self.idx, _I64), len(self.idx), self.dt, _p(params))
         assert rc == 0
 
     def one_step(self) -> None:

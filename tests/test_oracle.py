@@ -1,0 +1,245 @@
+"""Pins of the CPU oracle (no GPU).
+
+The reference ships no golden vectors; its own known-answer tests are the
+physiological windows of tests/test_models_2d.py / test_models_3d.py (APD and
+amplitude per model) and tests/test_trackers_2d.py (conduction velocity from
+the activation-time map).  They are reproduced here on the oracle.  The
+stencils are pinned by closed forms (interior and Neumann edge weights),
+invariants (weights sum to one, constant fields are fixed points, mirror and
+transpose symmetry) and special cases (axis-aligned fibres have no cross
+terms).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle.oracle import MODELS, OracleSim, OStim, OTracker, time_fold
+from tests.scenarios import boxed_mesh, random_fibers
+
+
+def _apd(u, dt, thr, beat=3):
+    # tests/test_models_2d.py:74-106
+    up = np.where((u[:-1] < thr) & (u[1:] >= thr))[0]
+    dn = np.where((u[:-1] > thr) & (u[1:] <= thr))[0]
+    if len(up) <= beat or len(dn) == 0:
+        return None
+    s = up[beat]
+    e = dn[dn > s]
+    return None if len(e) == 0 else (e[0] - s) * dt
+
+
+# (model, current, duration, t_calc, t_prebeats, threshold, APD window, amplitude checks)
+# tests/test_models_2d.py:108-293
+WINDOWS = [
+    ("aliev_panfilov", 5, 0.5, 80, 60, 0.1, (20, 25), ("approx_max", 1.0, 0.1)),
+    ("barkley", 5, 0.1, 80, 60, 0.1, (1, 4), ("approx_max", 1.0, 0.1)),
+    ("mitchell_schaeffer", 5, 0.5, 1000, 1000, 0.1, (250, 350), ("approx_max", 0.95, 0.1)),
+    ("fenton_karma", 5, 0.5, 1000, 1000, 0.1, (100, 200), ("approx_max", 1.0, 0.1)),
+    ("bueno_orovio", 5, 0.5, 1000, 1000, 0.1, (200, 300), ("approx_max", 1.4, 0.1)),
+    ("luo_rudy91", 100, 1, 1000, 1000, -70, (350, 400), ("gt_max", 20, -80)),
+    ("tp06", 100, 1, 1000, 1000, -70, (280, 320), ("gt_max", 20, -80)),
+    ("courtemanche", 100, 1, 1000, 1000, -70, (200, 300), ("gt_max", 10, -80)),
+]
+
+
+def run_reference_protocol(name, cur, dur, t_calc, t_pre, dim=2):
+    """tests/test_models_2d.py:8-72: 5x3 (x3) tissue, four current pulses on
+    the first two rows, u recorded at [3, 1] every step."""
+    shape = (5, 3) if dim == 2 else (5, 3, 3)
+    sim = OracleSim(name, boxed_mesh(shape), 0.01, 0.25)
+    region = (slice(0, 2), slice(0, shape[1])) + ((slice(0, shape[2]),) if dim == 3 else ())
+    for k in range(4):
+        sim.stims.append(OStim("current", k * t_pre, cur, region, duration=dur))
+    cell = [3, 1] if dim == 2 else [3, 1, 1]
+    tr = OTracker("probe", cells=cell)
+    sim.trackers.append(tr)
+    sim.run(3 * t_pre + t_calc)
+    return np.squeeze(np.array(tr.data))
+
+
+@pytest.mark.parametrize("case", WINDOWS, ids=[w[0] for w in WINDOWS])
+def test_reference_model_windows_2d(case):
+    name, cur, dur, t_calc, t_pre, thr, (lo, hi), amp = case
+    u = run_reference_protocol(name, cur, dur, t_calc, t_pre)
+    apd = _apd(u, 0.01, thr)
+    assert apd is not None and lo <= apd <= hi, apd
+    if amp[0] == "approx_max":
+        assert abs(np.max(u) - amp[1]) <= amp[2]
+        assert abs(np.min(u) - 0.0) <= 0.01
+    else:
+        assert np.max(u) > amp[1] and np.min(u) < amp[2]
+
+
+@pytest.mark.parametrize("name", ["aliev_panfilov", "barkley"])
+def test_reference_model_windows_3d(name):
+    # tests/test_models_3d.py: tighter max-amplitude tolerance (0.02)
+    case = [w for w in WINDOWS if w[0] == name][0]
+    _, cur, dur, t_calc, t_pre, thr, (lo, hi), amp = case
+    u = run_reference_protocol(name, cur, dur, t_calc, t_pre, dim=3)
+    apd = _apd(u, 0.01, thr)
+    assert lo <= apd <= hi
+    assert abs(np.max(u) - amp[1]) <= 0.1
+
+
+def test_activation_time_conduction_velocity():
+    # tests/test_trackers_2d.py:7-22, 123-146: AP cable 12x3, speed in [1.5, 2]
+    sim = OracleSim("aliev_panfilov", boxed_mesh((12, 3)), 0.01, 0.25)
+    sim.stims.append(OStim("current", 0, 5, (slice(0, 5), slice(0, 3)), duration=0.5))
+    tr = OTracker("act_time", threshold=0.5)
+    sim.trackers.append(tr)
+    sim.run(3)
+    speed = 5 * 0.25 / tr.data[10, 1]
+    assert 1.5 <= speed <= 2
+
+
+def test_resting_states_are_stable():
+    # with no stimulus the physiological models stay near rest for 200 ms
+    for name in ("luo_rudy91", "tp06", "courtemanche"):
+        sim = OracleSim(name, boxed_mesh((5, 3)), 0.01, 0.25)
+        tr = OTracker("probe", cells=[2, 1])
+        sim.trackers.append(tr)
+        sim.run(200)
+        u = np.squeeze(np.array(tr.data))
+        assert u.max() < -75 and u.min() > -95, (name, u.min(), u.max())
+    # the two-variable models at their rest state do not move at all
+    for name in ("aliev_panfilov", "barkley", "mitchell_schaeffer"):
+        sim = OracleSim(name, boxed_mesh((6, 6)), 0.01, 0.25)
+        sim.run(5)
+        assert np.all(sim.u == 0.0)
+
+
+# ---------------------------------------------------------------- stencils
+def _weights(shape, stencil, fibers=None, cond=1.0, fibrosis=0.0, D_ac=1 / 9, dr=0.25):
+    sim = OracleSim("aliev_panfilov", boxed_mesh(shape, fibrosis), 0.01, dr, fibers=fibers,
+                    conductivity=cond, stencil=stencil, D_ac=D_ac)
+    sim.compute_weights()
+    return sim
+
+
+@pytest.mark.parametrize("dim,stencil", [(2, "iso"), (3, "iso"), (2, "aniso"), (3, "aniso"),
+                                         (2, "sym")])
+def test_weights_sum_to_one_and_constant_fixed_point(dim, stencil):
+    shape = (12, 13) if dim == 2 else (9, 10, 11)
+    fib = random_fibers(shape) if stencil != "iso" else None
+    sim = _weights(shape, stencil, fibers=fib, fibrosis=0.15)
+    w = sim.weights.reshape(-1, sim.weights.shape[-1])[sim.idx]
+    assert np.max(np.abs(w.sum(axis=1) - 1.0)) < 1e-13
+    sim.u = np.full(shape, 0.7)
+    sim.u_new = np.zeros(shape)
+    sim._diffuse()
+    assert np.max(np.abs(sim.u_new.reshape(-1)[sim.idx] - 0.7)) < 1e-13
+    # fibrosis and empty nodes never receive weight
+    K = sim.weights.shape[-1]
+    assert np.all(sim.weights.reshape(-1, K)[np.flatnonzero(sim.mesh.reshape(-1) != 1)][:, [
+        q for q in range(K) if q != (2 if (dim, stencil) == (2, "iso") else
+                                     3 if (dim, stencil) == (3, "iso") else 4)]] == 0)
+
+
+def test_iso_closed_forms():
+    # interior weight r = D*dt/dr**2, centre 1-4r (2D) / 1-6r (3D); a node whose
+    # opposite neighbour is outside the tissue gets 2r (Neumann ghost doubling,
+    # isotropic_stencil_2d.py:158)
+    r = 1.0 * 0.01 / 0.25 ** 2
+    sim = _weights((7, 7), "iso")
+    w = sim.weights
+    assert np.allclose(w[3, 3], [r, r, 1 - 4 * r, r, r], rtol=0, atol=1e-15)
+    assert w[1, 3, 4] == pytest.approx(2 * r) and w[1, 3, 0] == 0.0
+    sim3 = _weights((7, 7, 7), "iso")
+    assert np.allclose(sim3.weights[3, 3, 3], [r, r, r, 1 - 6 * r, r, r, r], atol=1e-15)
+
+
+def test_aniso_axis_aligned_fibres_have_no_cross_terms():
+    shape = (9, 9)
+    f = np.zeros((*shape, 2))
+    f[..., 0] = 1.0
+    sim = _weights(shape, "aniso", fibers=f)
+    s = 0.01 / 0.25 ** 2
+    w = sim.weights[4, 4]
+    assert np.all(w[[0, 2, 6, 8]] == 0.0)
+    assert w[1] == pytest.approx(1.0 * s, rel=1e-15) and w[7] == pytest.approx(1.0 * s, rel=1e-15)
+    assert w[3] == pytest.approx(s / 9, rel=1e-14) and w[5] == pytest.approx(s / 9, rel=1e-14)
+
+
+def test_aniso_transpose_and_mirror_symmetry():
+    # transposing the grid and swapping the fibre components transposes the
+    # 3x3 weight block; mirroring i with f_x -> -f_x mirrors it
+    shape = (11, 11)
+    mesh = boxed_mesh(shape, 0.2, seed=5)
+    f = random_fibers(shape, seed=7)
+    a = OracleSim("aliev_panfilov", mesh, 0.01, 0.25, fibers=f)
+    a.compute_weights()
+    b = OracleSim("aliev_panfilov", mesh.T.copy(), 0.01, 0.25,
+                  fibers=np.ascontiguousarray(f.transpose(1, 0, 2)[..., ::-1]))
+    b.compute_weights()
+    wa = a.weights.reshape(*shape, 3, 3)
+    wb = b.weights.reshape(*shape, 3, 3)
+    inner = (slice(1, -1), slice(1, -1))
+    assert np.allclose(wa.transpose(1, 0, 3, 2)[inner], wb[inner], atol=1e-15)
+    fm = f[::-1].copy()
+    fm[..., 0] *= -1
+    c = OracleSim("aliev_panfilov", mesh[::-1].copy(), 0.01, 0.25, fibers=fm)
+    c.compute_weights()
+    wc = c.weights.reshape(*shape, 3, 3)
+    assert np.allclose(wa[::-1, :, ::-1, :][inner], wc[inner], atol=1e-15)
+
+
+def test_aniso3d_reduces_to_2d_in_a_plane():
+    # fibres in the xy-plane and a slab one tissue node thick in z: the
+    # 19-point weights restricted to the plane equal the 9-point weights
+    f3 = random_fibers((10, 11, 3), seed=2)
+    f3[..., 2] = 0.0
+    f3[:, :, 0] = f3[:, :, 1]
+    f3[:, :, 2] = f3[:, :, 1]
+    m3 = boxed_mesh((10, 11, 3))
+    s3 = OracleSim("aliev_panfilov", m3, 0.01, 0.25, fibers=f3)
+    s3.compute_weights()
+    s2 = OracleSim("aliev_panfilov", m3[:, :, 1].copy(), 0.01, 0.25,
+                   fibers=np.ascontiguousarray(f3[:, :, 1, :2]))
+    s2.compute_weights()
+    w3 = s3.weights[:, :, 1, :9]
+    assert np.allclose(w3[1:-1, 1:-1], s2.weights[1:-1, 1:-1], atol=1e-15)
+    assert np.all(s3.weights[:, :, 1, 9:] == 0.0)
+
+
+# ---------------------------------------------------------------- time and stimuli
+def test_time_fold_values():
+    # cardiac_model.py:176 accumulates t += dt
+    t = time_fold(0.0, 0.01, 31001)
+    assert t[31000] == 309.99999999986306
+    assert time_fold(0.0, 0.01, 1001)[1000] == 9.999999999999831
+
+
+def test_current_stimulus_fencepost():
+    # a 0.5 ms current at dt 0.01 is applied 51 times (stim.py:56)
+    sim = OracleSim("aliev_panfilov", boxed_mesh((5, 5)), 0.01, 0.25)
+    stim = OStim("current", 0.0, 1.0, (slice(2, 3), slice(2, 3)), duration=0.5)
+    count = {"n": 0}
+    orig = stim.apply
+
+    def counting(s):
+        count["n"] += 1
+        orig(s)
+
+    stim.apply = counting
+    sim.stims.append(stim)
+    sim.run(2.0)
+    assert count["n"] == 51
+
+
+def test_model_tables_match_the_product():
+    # the product's parameter/initial-value tables equal the oracle's
+    import finitewave_amd as fw
+    classes = {"aliev_panfilov": fw.AlievPanfilov2D, "barkley": fw.Barkley2D,
+               "mitchell_schaeffer": fw.MitchellSchaeffer2D, "fenton_karma": fw.FentonKarma2D,
+               "bueno_orovio": fw.BuenoOrovio2D, "luo_rudy91": fw.LuoRudy912D,
+               "tp06": fw.TP062D, "courtemanche": fw.Courtemanche2D}
+    for name, cls in classes.items():
+        m = cls()
+        spec = MODELS[name]
+        assert m._param_values() == [float(v) for _, v in spec["params"]], name
+        assert m._state_call_order() == [n for n, _ in spec["state"]], name
+        assert m.D_model == spec["D"] and m.init_u == spec["init_u"], name
+        inits = dict(spec["state"])
+        for attr, init_attr, _ in cls._state:
+            assert getattr(m, init_attr) == inits[attr], (name, attr)
