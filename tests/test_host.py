@@ -1,0 +1,161 @@
+"""Host-side logic without a GPU: the C-ABI library's symbol table, tissue
+codes, half-step fields, field compression, tracker/saver gates, stimulus
+boxes and the failure mode when no device is present."""
+from __future__ import annotations
+
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import finitewave_amd as fw
+from finitewave_amd import _native
+from finitewave_amd.model import _gate_open, _saver_due
+from finitewave_amd.stencil import compress_field
+from tests.scenarios import boxed_mesh, layered_fibers, random_fibers
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    hdr = (ROOT / "include" / "finitewave_amd.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|const char\*)\s+(fw_[a-z0-9_]+)\s*\(", hdr, re.M))
+    assert declared == set(_native.EXPORTS)
+    L = _native.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert L.fw_abi_version() == 1
+    assert _native.model_info(_native.MODEL_IDS["tp06"]) == (19, 49)
+    assert _native.model_info(_native.MODEL_IDS["courtemanche"]) == (21, 39)
+    # TP06's cai/casr/cass/nai/Ki stay FP64 on the FP32 path
+    assert [_native.state_is_f64(6, 1, k) for k in range(7)] == [True] * 5 + [False] * 2
+    assert all(_native.state_is_f64(6, 0, k) for k in range(19))
+
+
+def test_abi_rejects_bad_arguments_without_a_device():
+    L = _native.lib()
+    desc = _native.fw_plan_desc()
+    desc.ndim = 4
+    import ctypes
+    h = ctypes.c_void_p()
+    assert L.fw_plan_create(ctypes.byref(desc), ctypes.byref(h)) == -1
+    assert b"ndim" in L.fw_last_error()
+    assert L.fw_model_info(99, None, None) == -1
+
+
+def test_tissue_boundaries_and_codes():
+    t = fw.CardiacTissue2D([6, 7])
+    assert t.mesh[0].sum() == 0 and t.mesh[:, -1].sum() == 0
+    t.mesh[2, 3] = 2
+    special = np.zeros((6, 7), dtype=np.int8)
+    special[3, 3] = 1
+    t.special_boundaries = special
+    code = t.node_codes()
+    assert code[2, 3] == 0 and code[3, 3] == 3 and code[1, 1] == 1
+    t.compute_myo_indexes()
+    assert np.ravel_multi_index((3, 3), (6, 7)) not in t.myo_indexes
+    t.mesh[0, 2] = 1  # in-place edit past the setter
+    with pytest.raises(ValueError):
+        t.check_boundary()
+    t3 = fw.CardiacTissue3D([4, 5, 6])
+    assert t3.mesh.sum() == 2 * 3 * 4
+
+
+def test_half_step_fields_match_oracle_bitwise():
+    from oracle.oracle import half_step_fields
+    for shape, fib in [((9, 10), random_fibers((9, 10))),
+                       ((6, 7, 8), layered_fibers((6, 7, 8)))]:
+        t = (fw.CardiacTissue2D if len(shape) == 2 else fw.CardiacTissue3D)(list(shape))
+        t.fibers = fib
+        t.conductivity = 0.5 + np.random.RandomState(0).random_sample(shape)
+        st = fw.AsymmetricStencil2D() if len(shape) == 2 else fw.AsymmetricStencil3D()
+        m = fw.AlievPanfilov2D()
+        mine = st.half_step_fields(m, t)
+        ref = half_step_fields(1, 1 / 9, fib, t.conductivity, shape, len(shape))
+        assert len(mine) == len(ref)
+        for a, b in zip(mine, ref):
+            assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+def test_compress_field_modes():
+    assert compress_field(np.full((4, 5), 0.3))[:3] == (_native.FIELD_UNIFORM, 0, 0.3)
+    a = np.tile(np.arange(6.0), (4, 5, 1))
+    mode, axis, _, data = compress_field(a)
+    assert (mode, axis) == (_native.FIELD_AXIS, 2) and np.array_equal(data, np.arange(6.0))
+    b = np.random.RandomState(1).random_sample((4, 5))
+    assert compress_field(b)[0] == _native.FIELD_FULL
+    # signed zeros are different bit patterns: no silent merge
+    z = np.zeros((3, 3))
+    z[1, 1] = -0.0
+    assert compress_field(z)[0] == _native.FIELD_FULL
+
+
+def test_tracker_and_saver_gates():
+    tr = fw.ActivationTime2DTracker()
+    tr.start_time, tr.end_time, tr.step = 1.0, 2.0, 3
+    assert not _gate_open(tr, 0.5, 3)
+    assert _gate_open(tr, 1.5, 6) and not _gate_open(tr, 1.5, 7)
+    assert not _gate_open(tr, 2.5, 6)
+
+    class M:
+        t, t_max = 4.0, 5.0
+    s = fw.StateSaver("x", time=3)
+    s.passed = False
+    assert _saver_due(s, M)
+    s2 = fw.StateSaver("x", time=-1)
+    assert not _saver_due(s2, M)
+    col = fw.StateSaverCollection()
+    col.savers = [s2, s]
+    assert _saver_due(col, M)
+
+
+def test_stim_box_descriptors_clip_like_numpy_slices():
+    from finitewave_amd.stimulation import _box_desc
+    d = _box_desc(1, (slice(0, 2), slice(-3, None)), (5, 7), 0.5, u_max=1.0)
+    assert list(d.lo[:3]) == [0, 4, 0] and list(d.hi[:3]) == [2, 7, 1]
+    assert d.clamp == 1 and d.value == 0.5
+
+
+def test_state_saver_and_loader_roundtrip(tmp_path):
+    class M:
+        state_vars = ["u", "v"]
+        t, t_max = 1.0, 1.0
+    m = M()
+    m.u = np.arange(6.0).reshape(2, 3)
+    m.v = -m.u
+    s = fw.StateSaver(str(tmp_path / "st"))
+    s.initialize(m)
+    s.save()
+    n = M()
+    ld = fw.StateLoader(str(tmp_path / "st"))
+    ld.initialize(n)
+    ld.load()
+    assert np.array_equal(n.u, m.u) and np.array_equal(n.v, m.v)
+    with pytest.raises(FileNotFoundError):
+        fw.StateLoader(str(tmp_path / "missing")).initialize(n)
+
+
+def test_run_without_device_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a device is present")
+    model = fw.AlievPanfilov2D()
+    model.dt, model.dr, model.t_max = 0.01, 0.25, 0.1
+    model.cardiac_tissue = fw.CardiacTissue2D([8, 8])
+    model.prog_bar = False
+    with pytest.raises(fw.NativeUnavailableError):
+        model.run()
+
+
+def test_model_classes_keep_reference_attributes():
+    m = fw.TP062D()
+    assert m.state_vars[0] == "u" and len(m.state_vars) == 20
+    assert m.D_model == 0.154 and m.init_Ki == 138.3 and m.knaca == 1000
+    c = fw.Courtemanche3D()
+    assert c.Vj == c.Vc * 0.68 and c.state_vars[15:17] == ["xr", "xs"]
+    assert isinstance(fw.AlievPanfilov3D().select_stencil(fw.CardiacTissue3D([4, 4, 4])),
+                      fw.IsotropicStencil3D)
+    t = fw.CardiacTissue2D([4, 4])
+    t.fibers = np.zeros((4, 4, 2))
+    assert isinstance(fw.Barkley2D().select_stencil(t), fw.AsymmetricStencil2D)
