@@ -52,7 +52,7 @@ struct KArgs {
   uint32_t per_xcd;  // logical blocks per XCD
   Field f[9];
   const double* __restrict__ w;  // precomputed weights (stencil 2)
-  double D, dtd, dr2;
+  double D, dtd, dr2, inv_dr2;  // inv_dr2: exact 1/dr2 when dr2 = 2^k, else 0
   R dt, s;
   Params P;
   int32_t* act;
@@ -106,12 +106,19 @@ __device__ __forceinline__ void minor_exact(double d, int m0, int m1, int m2,
   q[5] = m5 ? d / dmh : 0.0;
 }
 
-// numpy epilogue: w = w * D * dt / dr**2 ; w[..., centre] += 1
+// numpy epilogue: w = w * D * dt / dr**2 ; w[..., centre] += 1.  When dr**2
+// is a power of two, inv_dr2 != 0 holds its exact reciprocal and the division
+// is the same multiplication (barring subnormal results).
 template <int K>
 __device__ __forceinline__ void scale_exact(double W[K], int centre, double D,
-                                            double dt, double dr2) {
+                                            double dt, double dr2, double inv_dr2) {
+  if (inv_dr2 != 0.0) {
 #pragma unroll
-  for (int q = 0; q < K; ++q) W[q] = W[q] * D * dt / dr2;
+    for (int q = 0; q < K; ++q) W[q] = W[q] * D * dt * inv_dr2;
+  } else {
+#pragma unroll
+    for (int q = 0; q < K; ++q) W[q] = W[q] * D * dt / dr2;
+  }
   W[centre] += 1.0;
 }
 
@@ -310,7 +317,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
         W[3] = iso_comp(dE, mE, mW);
         W[4] = iso_comp(dS, mS, mN);
         W[2] = -(W[0] + W[1] + W[3] + W[4]);
-        scale_exact<5>(W, 2, a.D, a.dtd, a.dr2);
+        scale_exact<5>(W, 2, a.D, a.dtd, a.dr2, a.inv_dr2);
         return u[c - sN] * W[0] + u[c - 1] * W[1] + uc * W[2] + u[c + 1] * W[3] +
                u[c + sN] * W[4];
       } else {
@@ -342,7 +349,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
       if constexpr (sizeof(R) == 8) {
         double W[9];
         wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
-        scale_exact<9>(W, 4, a.D, a.dtd, a.dr2);
+        scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
         return u[c - sN - 1] * W[0] + u[c - sN] * W[1] + u[c - sN + 1] * W[2] +
                u[c - 1] * W[3] + uc * W[4] + u[c + 1] * W[5] +
                u[c + sN - 1] * W[6] + u[c + sN] * W[7] + u[c + sN + 1] * W[8];
@@ -407,7 +414,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
         W[5] = iso_comp(dYp, mYp, mYm);
         W[6] = iso_comp(dXp, mXp, mXm);
         W[3] = -(W[0] + W[1] + W[2] + W[4] + W[5] + W[6]);
-        scale_exact<7>(W, 3, a.D, a.dtd, a.dr2);
+        scale_exact<7>(W, 3, a.D, a.dtd, a.dr2, a.inv_dr2);
         return u[c - si] * W[0] + u[c - sj] * W[1] + u[c - 1] * W[2] + uc * W[3] +
                u[c + 1] * W[4] + u[c + sj] * W[5] + u[c + si] * W[6];
       } else {
@@ -455,7 +462,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
       if constexpr (sizeof(R) == 8) {
         double W[19];
         wts_aniso3d(W, M, d);
-        scale_exact<19>(W, 4, a.D, a.dtd, a.dr2);
+        scale_exact<19>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
         return u[c - si - sj] * W[0] + u[c - si] * W[1] + u[c - si + sj] * W[2] +
                u[c - sj] * W[3] + uc * W[4] + u[c + sj] * W[5] +
                u[c + si - sj] * W[6] + u[c + si] * W[7] + u[c + si + sj] * W[8] +

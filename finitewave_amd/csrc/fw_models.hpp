@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "fw_math.hpp"
+
 namespace fw {
 
 constexpr int kBlock = 256;  // work-items per block of the node-parallel kernels
@@ -28,16 +30,27 @@ struct StatePtrs {
 };
 
 // ---- math dispatch --------------------------------------------------------
-__device__ __forceinline__ double fexp(double x) { return exp(x); }
-__device__ __forceinline__ float fexp(float x) { return expf(x); }
-__device__ __forceinline__ double flog(double x) { return log(x); }
-__device__ __forceinline__ float flog(float x) { return logf(x); }
+// FP64: the table-driven exp/log of fw_math.hpp, correctly rounded except in
+// rare near-halfway cases, so the kernel's last bits follow the host libm the
+// reference's numba kernels call; FP32: the device library.
+__device__ __forceinline__ double fexp(double x) { return dd_exp(x); }
+__device__ __forceinline__ float fexp(float x) { return __expf(x); }
+__device__ __forceinline__ double flog(double x) { return dd_log(x); }
+__device__ __forceinline__ float flog(float x) { return __logf(x); }
 __device__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
 __device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
 __device__ __forceinline__ double ftanh(double x) { return tanh(x); }
 __device__ __forceinline__ float ftanh(float x) { return tanhf(x); }
 __device__ __forceinline__ double fpow(double x, double y) { return pow(x, y); }
 __device__ __forceinline__ float fpow(float x, float y) { return powf(x, y); }
+
+// (e^y - 1)/y, continuous at y = 0.  The FP32 path evaluates the rate
+// expressions of the form x / (e^(k x) - 1) through it: in float e^(k x)
+// rounds to exactly 1 within about 1e-7 of the removable singularity and the
+// literal form divides by zero there.
+__device__ __forceinline__ float exprel(float y) {
+  return fabsf(y) < 1e-4f ? 1.0f + 0.5f * y : expm1f(y) / y;
+}
 
 template <typename T>
 __device__ __forceinline__ T& ref(const StatePtrs& S, int k, int64_t c) {
@@ -206,8 +219,11 @@ __device__ __forceinline__ R ionic_lr91(R u, R un, const StatePtrs& S,
                  R(3.474 * 1e-05) * fexp(R(-0.04391) * u)) *
                 (u + R(37.78)) / (R(1) + fexp(R(0.311) * (u + R(79.23))));
     }
-    const R alpha_m =
-        R(0.32) * (u + R(47.13)) / (R(1) - fexp(R(-0.1) * (u + R(47.13))));
+    R alpha_m;
+    if constexpr (sizeof(R) == 4)
+      alpha_m = R(3.2) / exprel(R(-0.1) * (u + R(47.13)));
+    else
+      alpha_m = R(0.32) * (u + R(47.13)) / (R(1) - fexp(R(-0.1) * (u + R(47.13))));
     const R beta_m = R(0.08) * fexp(-u / R(11));
     const R m = lr91_gate(mr, dt, alpha_m, beta_m);
     const R h = lr91_gate(hr, dt, alpha_h, beta_h);
@@ -245,8 +261,11 @@ __device__ __forceinline__ R ionic_lr91(R u, R un, const StatePtrs& S,
     const R G_K = gk * fsqrt(ko / R(5.4));
     R Xi;
     if (u > R(-100)) {
-      Xi = R(2.837) * (fexp(R(0.04) * (u + R(77))) - R(1)) /
-           ((u + R(77)) * fexp(R(0.04) * (u + R(35))));
+      if constexpr (sizeof(R) == 4)
+        Xi = R(2.837 * 0.04) * exprel(R(0.04) * (u + R(77))) / fexp(R(0.04) * (u + R(35)));
+      else
+        Xi = R(2.837) * (fexp(R(0.04) * (u + R(77))) - R(1)) /
+             ((u + R(77)) * fexp(R(0.04) * (u + R(35))));
     } else {
       Xi = R(1);
     }
@@ -390,9 +409,16 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
     fr = f;
     f2r = f2;
     fcr = fcass;
-    ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * (F * F / (R_ * T)) *
-           (R(0.25) * fexp(R(2) * (u - R(15)) * F / (R_ * T)) * cass - cao) /
-           (fexp(R(2) * (u - R(15)) * F / (R_ * T)) - R(1.0));
+    if constexpr (sizeof(R) == 4) {
+      // (u - 15) / (e^X - 1) = 1 / ((2F/RT) * exprel(X)),  X = 2(u - 15)F/RT
+      const R X = R(2) * (u - R(15)) * F / (R_ * T);
+      ical = gcal * d * f * f2 * fcass * R(4) * (F * F / (R_ * T)) *
+             (R(0.25) * fexp(X) * cass - cao) / (R(2) * F / (R_ * T) * exprel(X));
+    } else {
+      ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * (F * F / (R_ * T)) *
+             (R(0.25) * fexp(R(2) * (u - R(15)) * F / (R_ * T)) * cass - cao) /
+             (fexp(R(2) * (u - R(15)) * F / (R_ * T)) - R(1.0));
+    }
   }
   // calc_ito
   R ito;
@@ -554,10 +580,14 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   {
     R& mr = ref<R>(S, 5, c);
     R am;
-    if (u == R(-47.13)) {
-      am = R(3.2);
+    if constexpr (sizeof(R) == 4) {
+      am = R(3.2) / exprel(R(-0.1) * (u + R(47.13)));
     } else {
-      am = R(0.32) * (u + R(47.13)) / (R(1) - fexp(R(-0.1) * (u + R(47.13))));
+      if (u == R(-47.13)) {
+        am = R(3.2);
+      } else {
+        am = R(0.32) * (u + R(47.13)) / (R(1) - fexp(R(-0.1) * (u + R(47.13))));
+      }
     }
     const R bm = R(0.08) * fexp(-u / R(11));
     const R m_inf = am / (am + bm);
@@ -642,8 +672,14 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   {
     R& xrr = ref<R>(S, 15, c);
     const R gkr = R(0.0294);
-    const R axr = R(0.0003) * (u + R(14.1)) / (R(1) - fexp(-(u + R(14.1)) / R(5)));
-    const R bxr = R(7.3898e-05) * (u - R(3.3328)) / (fexp((u - R(3.3328)) / R(5.1237)) - R(1));
+    R axr, bxr;
+    if constexpr (sizeof(R) == 4) {
+      axr = R(0.0003 * 5) / exprel(-(u + R(14.1)) / R(5));
+      bxr = R(7.3898e-05 * 5.1237) / exprel((u - R(3.3328)) / R(5.1237));
+    } else {
+      axr = R(0.0003) * (u + R(14.1)) / (R(1) - fexp(-(u + R(14.1)) / R(5)));
+      bxr = R(7.3898e-05) * (u - R(3.3328)) / (fexp((u - R(3.3328)) / R(5.1237)) - R(1));
+    }
     const R tau_xr = R(1) / (axr + bxr);
     const R xr_inf = R(1) / (R(1) + fexp(-(u + R(14.1)) / R(6.5)));
     const R xr = crn_gate(xrr, xr_inf, tau_xr, dt);
@@ -654,8 +690,14 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   R iks;
   {
     R& xsr = ref<R>(S, 14, c);
-    const R axs = R(4e-05) * (u - R(19.9)) / (R(1) - fexp(-(u - R(19.9)) / R(17)));
-    const R bxs = R(3.5e-05) * (u - R(19.9)) / (fexp((u - R(19.9)) / R(9)) - R(1));
+    R axs, bxs;
+    if constexpr (sizeof(R) == 4) {
+      axs = R(4e-05 * 17) / exprel(-(u - R(19.9)) / R(17));
+      bxs = R(3.5e-05 * 9) / exprel((u - R(19.9)) / R(9));
+    } else {
+      axs = R(4e-05) * (u - R(19.9)) / (R(1) - fexp(-(u - R(19.9)) / R(17)));
+      bxs = R(3.5e-05) * (u - R(19.9)) / (fexp((u - R(19.9)) / R(9)) - R(1));
+    }
     const R tau_xs = R(1) / (R(2) * (axs + bxs));
     const R xs_inf = R(1) / fsqrt(R(1) + fexp(-(u - R(19.9)) / R(12.7)));
     const R xs = crn_gate(xsr, xs_inf, tau_xs, dt);
@@ -668,8 +710,14 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
     R& dr = ref<R>(S, 8, c);
     R& fr = ref<R>(S, 9, c);
     R& fcar = ref<R>(S, 16, c);
-    const R tau_d = (R(1) - fexp(-(u + R(10)) / R(6.24))) /
-                    (R(0.035) * (u + R(10)) * (R(1) + fexp(-(u + R(10)) / R(6.24))));
+    R tau_d;
+    if constexpr (sizeof(R) == 4) {
+      const R y = -(u + R(10)) / R(6.24);
+      tau_d = exprel(y) / (R(6.24 * 0.035) * (R(1) + fexp(y)));
+    } else {
+      tau_d = (R(1) - fexp(-(u + R(10)) / R(6.24))) /
+              (R(0.035) * (u + R(10)) * (R(1) + fexp(-(u + R(10)) / R(6.24))));
+    }
     const R d_inf = R(1) / (R(1) + fexp(-(u + R(10)) / R(8.0)));
     const R tau_f = R(9) / (R(0.0197) * fexp(-R(0.0337 * 0.0337) * ((u + R(10)) * (u + R(10)))) +
                             R(0.02));
@@ -716,8 +764,14 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
     const R u_inf = R(1) / (R(1) + fexp(-(Fn - R(3.4175e-13)) / R(1.367e-15)));
     const R tau_v = R(1.91) + R(2.09) / (R(1) + fexp(-(Fn - R(3.4175e-13)) / R(1.367e-15)));
     const R v_inf = R(1) - R(1) / (R(1) + fexp(-(Fn - R(6.835e-14)) / R(1.367e-15)));
-    const R tau_w = R(6) * (R(1) - fexp(-(u - R(7.9)) / R(5.0))) /
-                    ((R(1) + R(0.3) * fexp(-(u - R(7.9)) / R(5.0))) * (u - R(7.9)));
+    R tau_w;
+    if constexpr (sizeof(R) == 4) {
+      const R y = -(u - R(7.9)) / R(5.0);
+      tau_w = R(6.0 / 5.0) * exprel(y) / (R(1) + R(0.3) * fexp(y));
+    } else {
+      tau_w = R(6) * (R(1) - fexp(-(u - R(7.9)) / R(5.0))) /
+              ((R(1) + R(0.3) * fexp(-(u - R(7.9)) / R(5.0))) * (u - R(7.9)));
+    }
     const R w_inf = R(1) - R(1) / (R(1) + fexp(-(u - R(40)) / R(17.0)));
     const R urel = crn_gate(urr, u_inf, tau_u, dt);
     const R vrel = crn_gate(vrr, v_inf, tau_v, dt);

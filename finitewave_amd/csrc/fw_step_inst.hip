@@ -57,6 +57,11 @@ int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
   a.D = d.D_model;
   a.dtd = d.dt;
   a.dr2 = d.dr2;
+  {
+    int e2;
+    const double mant = frexp(d.dr2, &e2);
+    a.inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
+  }
   a.dt = (Real)d.dt;
   a.s = (Real)(d.D_model * d.dt / d.dr2);
   for (int q = 0; q < 64; ++q) a.P.v[q] = q < d.n_params ? d.params[q] : 0.0;

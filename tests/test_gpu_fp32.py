@@ -1,0 +1,104 @@
+"""Accuracy of the FP32 path against the FP64 path.
+
+The FP64 path is the oracle-pinned reference arithmetic (per-step |dV| <=
+1e-12 against the oracle, tests/test_gpu_parity.py), so it serves as the
+FP64 reference here for runs too long for the CPU oracle.  Tolerances stated
+activation-time error <= 1 dt, APD90 error
+<= 1 ms; the short-horizon V error is reported and bounded loosely.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import finitewave_amd as fw
+
+
+def _apd90(u, dt, threshold_up):
+    # APD90 as in the reference's restitution tutorial (Tutorials/RestitutionCurve.ipynb):
+    # last upstroke, 90 % repolarisation level between peak and minimum after it
+    up = np.where((u[:-1] < threshold_up) & (u[1:] >= threshold_up))[0]
+    if len(up) == 0:
+        return None
+    s = up[-1]
+    peak = np.max(u[s:])
+    level = peak - 0.9 * (peak - np.min(u[s:]))
+    idx = np.where(u[s:] < level)[0]
+    return None if len(idx) == 0 else idx[0] * dt
+
+
+def _planar(cls, precision, shape, t_max, cur, dur, thr, cell, fibers=None):
+    tissue = fw.CardiacTissue2D(list(shape))
+    tissue.fibers = fibers
+    model = cls()
+    model.dt, model.dr, model.t_max = 0.01, 0.25, t_max
+    model.precision = precision
+    model.cardiac_tissue = tissue
+    model.prog_bar = False
+    seq = fw.StimSequence()
+    seq.add_stim(fw.StimCurrentCoord2D(0, cur, dur, 0, shape[0], 0, 6))
+    model.stim_sequence = seq
+    act = fw.ActivationTime2DTracker()
+    act.threshold = thr
+    ap = fw.ActionPotential2DTracker()
+    ap.cell_ind = cell
+    ts = fw.TrackerSequence()
+    ts.add_tracker(act)
+    ts.add_tracker(ap)
+    model.tracker_sequence = ts
+    model.run()
+    return model, act.output, ap.output
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("aniso", [False, True])
+def test_tp06_fp32_activation_and_apd90(gpu, aniso):
+    shape = (40, 200)
+    fibers = None
+    if aniso:
+        fibers = np.zeros((*shape, 2))
+        fibers[..., 0] = np.cos(np.deg2rad(30))
+        fibers[..., 1] = np.sin(np.deg2rad(30))
+    out = {}
+    for prec in ("fp64", "fp32"):
+        out[prec] = _planar(fw.TP062D, prec, shape, 400.0, 100, 1.0, -40, [20, 150], fibers)
+    m64, act64, ap64 = out["fp64"]
+    m32, act32, ap32 = out["fp32"]
+    assert np.isfinite(m32.u).all()
+    hit = act64 >= 0
+    assert hit.sum() > 0.9 * (shape[0] - 2) * (shape[1] - 2)
+    assert np.array_equal(hit, act32 >= 0)
+    err_at = float(np.max(np.abs(act64[hit] - act32[hit])))
+    assert err_at <= 0.01 + 1e-9, f"activation-time error {err_at} ms > 1 dt"
+    a64, a32 = _apd90(ap64, 0.01, -40), _apd90(ap32, 0.01, -40)
+    assert a64 is not None and abs(a64 - a32) <= 1.0, (a64, a32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cls,thr", [(fw.AlievPanfilov2D, 0.5), (fw.MitchellSchaeffer2D, 0.5)])
+def test_fp32_two_variable_models(gpu, cls, thr):
+    shape = (30, 160)
+    t_max = 60.0 if cls is fw.AlievPanfilov2D else 400.0
+    out = {p: _planar(cls, p, shape, t_max, 5, 0.5, thr, [15, 120]) for p in ("fp64", "fp32")}
+    act64, act32 = out["fp64"][1], out["fp32"][1]
+    hit = act64 >= 0
+    assert hit.sum() > 0
+    assert np.array_equal(hit, act32 >= 0)
+    assert float(np.max(np.abs(act64[hit] - act32[hit]))) <= 0.01 + 1e-9
+    a64, a32 = _apd90(out["fp64"][2], 0.01, thr), _apd90(out["fp32"][2], 0.01, thr)
+    assert a64 is not None and abs(a64 - a32) <= 1.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("cls", [fw.LuoRudy912D, fw.Courtemanche2D, fw.FentonKarma2D,
+                                 fw.BuenoOrovio2D, fw.Barkley2D])
+def test_fp32_other_models_stay_close(gpu, cls):
+    # short horizon (5 ms) V error after an upstroke, reported as a bound
+    shape = (24, 60)
+    phys = cls in (fw.LuoRudy912D, fw.Courtemanche2D)
+    cur, thr = (100, -40) if phys else (5, 0.5)
+    out = {p: _planar(cls, p, shape, 5.0, cur, 1.0, thr, [12, 30]) for p in ("fp64", "fp32")}
+    u64, u32 = out["fp64"][0].u, out["fp32"][0].u
+    assert np.isfinite(u32).all()
+    scale = max(1.0, float(np.max(np.abs(u64))))
+    assert float(np.max(np.abs(u64 - u32))) <= 2e-3 * scale
