@@ -28,7 +28,8 @@ EXPORTS = (
     "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64",
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
-    "fw_stim_apply", "fw_act_time_update",
+    "fw_stim_apply", "fw_act_time_update", "fw_batch_create", "fw_batch_destroy",
+    "fw_batch_set_tissue", "fw_batch_set_blocks", "fw_batch_run", "fw_batch_stim",
 )
 
 

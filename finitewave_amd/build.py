@@ -40,9 +40,10 @@ def _units(build_dir: Path):
              (CSRC / "fw_literal.hip", build_dir / "fw_literal.o", FP64, [])]
     for prec in (0, 1):
         for model in range(8):
-            units.append((CSRC / "fw_step_inst.hip", build_dir / f"fw_step_p{prec}_m{model}.o",
-                          FP64 if prec == 0 else FP32,
-                          [f"-DFW_PREC={prec}", f"-DFW_MODEL={model}"]))
+            for stem in ("fw_step_inst", "fw_batch_inst"):
+                units.append((CSRC / f"{stem}.hip", build_dir / f"{stem}_p{prec}_m{model}.o",
+                              FP64 if prec == 0 else FP32,
+                              [f"-DFW_PREC={prec}", f"-DFW_MODEL={model}"]))
     return units
 
 

@@ -397,3 +397,245 @@ int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr, i
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- batches
+namespace fw {
+
+#define FW_DECL_BLAUNCHER(p, m) \
+  int fw_batch_launch_p##p##_m##m(const BatchLaunch&, hipStream_t);
+FW_DECL_BLAUNCHER(0, 0) FW_DECL_BLAUNCHER(0, 1) FW_DECL_BLAUNCHER(0, 2) FW_DECL_BLAUNCHER(0, 3)
+FW_DECL_BLAUNCHER(0, 4) FW_DECL_BLAUNCHER(0, 5) FW_DECL_BLAUNCHER(0, 6) FW_DECL_BLAUNCHER(0, 7)
+FW_DECL_BLAUNCHER(1, 0) FW_DECL_BLAUNCHER(1, 1) FW_DECL_BLAUNCHER(1, 2) FW_DECL_BLAUNCHER(1, 3)
+FW_DECL_BLAUNCHER(1, 4) FW_DECL_BLAUNCHER(1, 5) FW_DECL_BLAUNCHER(1, 6) FW_DECL_BLAUNCHER(1, 7)
+
+BatchLauncher batch_launcher(int precision, int model) {
+  static const BatchLauncher table[2][8] = {
+      {fw_batch_launch_p0_m0, fw_batch_launch_p0_m1, fw_batch_launch_p0_m2, fw_batch_launch_p0_m3,
+       fw_batch_launch_p0_m4, fw_batch_launch_p0_m5, fw_batch_launch_p0_m6, fw_batch_launch_p0_m7},
+      {fw_batch_launch_p1_m0, fw_batch_launch_p1_m1, fw_batch_launch_p1_m2, fw_batch_launch_p1_m3,
+       fw_batch_launch_p1_m4, fw_batch_launch_p1_m5, fw_batch_launch_p1_m6, fw_batch_launch_p1_m7}};
+  if (precision < 0 || precision > 1 || model < 0 || model > 7) return nullptr;
+  return table[precision][model];
+}
+
+}  // namespace fw
+
+static BatchLaunch batch_args(const fw_batch* b, int32_t step, const int32_t* in, int32_t* out) {
+  BatchLaunch L;
+  L.table = b->table;
+  L.btis = b->btis;
+  L.bstart = b->bstart;
+  L.nblocks = (uint32_t)b->nblocks;
+  L.per_xcd = (uint32_t)((b->nblocks + kXcds - 1) / kXcds);
+  L.step_in = in;
+  L.step_out = out;
+  L.step = step;
+  L.D = b->D;
+  L.dt = b->dt;
+  L.dr2 = b->dr2;
+  L.inv_dr2 = b->inv_dr2;
+  L.P = b->P;
+  return L;
+}
+
+extern "C" {
+
+int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt, double dr2,
+                    const double* params, int32_t n_params, int32_t max_tissues, fw_batch** out) {
+  if (!out || !params || model < 0 || model >= FW_N_MODELS || max_tissues <= 0 ||
+      (precision != FW_PREC_FP64 && precision != FW_PREC_FP32) || !(dt > 0) || !(dr2 > 0)) {
+    set_error("fw_batch_create: bad arguments");
+    return FW_E_INVALID;
+  }
+  if (n_params != model_n_params(model)) {
+    set_error("fw_batch_create: model %d takes %d parameters (got %d)", model,
+              model_n_params(model), n_params);
+    return FW_E_INVALID;
+  }
+  *out = nullptr;
+  fw_batch* b = new (std::nothrow) fw_batch();
+  if (!b) return FW_E_NOMEM;
+  b->model = model;
+  b->precision = precision;
+  b->max_tissues = max_tissues;
+  b->D = D_model;
+  b->dt = dt;
+  b->dr2 = dr2;
+  int e2;
+  const double mant = frexp(dr2, &e2);
+  b->inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
+  for (int q = 0; q < FW_MAX_PARAMS; ++q) b->P.v[q] = q < n_params ? params[q] : 0.0;
+  b->host = new (std::nothrow) fw_tissue_desc[max_tissues]();
+  if (!b->host || hipMalloc(&b->table, sizeof(fw_tissue_desc) * max_tissues) != hipSuccess ||
+      hipMalloc(&b->step_words, 2 * sizeof(int32_t)) != hipSuccess) {
+    set_error("fw_batch_create: allocation failed");
+    delete[] b->host;
+    delete b;
+    return FW_E_NOMEM;
+  }
+  if (hipMemset(b->table, 0, sizeof(fw_tissue_desc) * max_tissues) != hipSuccess) {
+    set_error("fw_batch_create: table init failed");
+    (void)fw_batch_destroy(b);
+    return FW_E_HIP;
+  }
+  *out = b;
+  return FW_OK;
+}
+
+int fw_batch_destroy(fw_batch* b) {
+  if (!b) return FW_OK;
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->table) (void)hipFree(b->table);
+  if (b->btis) (void)hipFree(b->btis);
+  if (b->bstart) (void)hipFree(b->bstart);
+  if (b->step_words) (void)hipFree(b->step_words);
+  delete[] b->host;
+  delete b;
+  return FW_OK;
+}
+
+int fw_batch_set_tissue(fw_batch* b, int32_t slot, const fw_tissue_desc* d, void* stream) {
+  if (!b || !d || slot < 0 || slot >= b->max_tissues) {
+    set_error("fw_batch_set_tissue: bad arguments");
+    return FW_E_INVALID;
+  }
+  if (d->n != d->ni * d->nj || d->ni < 3 || d->nj < 3 || !d->u || !d->un || !d->code ||
+      (d->stencil != FW_STENCIL_ISO && d->stencil != FW_STENCIL_ANISO)) {
+    set_error("fw_batch_set_tissue: malformed descriptor for slot %d", slot);
+    return FW_E_INVALID;
+  }
+  for (int k = 0; k < model_n_state(b->model); ++k)
+    if (!d->state[k]) {
+      set_error("fw_batch_set_tissue: state %d of slot %d is null", k, slot);
+      return FW_E_INVALID;
+    }
+  b->host[slot] = *d;
+  hipStream_t s = as_stream(stream);
+  if (hipMemcpyAsync(b->table + slot, b->host + slot, sizeof(fw_tissue_desc),
+                     hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("fw_batch_set_tissue: copy failed");
+    return FW_E_HIP;
+  }
+  return FW_OK;
+}
+
+int fw_batch_set_blocks(fw_batch* b, const int32_t* btis, const int32_t* bstart,
+                        int32_t nblocks, void* stream) {
+  if (!b || nblocks < 0 || (nblocks && (!btis || !bstart))) {
+    set_error("fw_batch_set_blocks: bad arguments");
+    return FW_E_INVALID;
+  }
+  for (int32_t q = 0; q < nblocks; ++q) {
+    if (btis[q] < 0 || btis[q] >= b->max_tissues || bstart[q] < 0 ||
+        (int64_t)bstart[q] >= b->host[btis[q]].n) {
+      set_error("fw_batch_set_blocks: block %d points outside its tissue", q);
+      return FW_E_INVALID;
+    }
+  }
+  hipStream_t s = as_stream(stream);
+  if (nblocks > b->cap_blocks) {
+    if (hipStreamSynchronize(s) != hipSuccess) return FW_E_HIP;
+    if (b->btis) (void)hipFree(b->btis);
+    if (b->bstart) (void)hipFree(b->bstart);
+    b->btis = b->bstart = nullptr;
+    const int32_t cap = nblocks + nblocks / 4 + 64;
+    if (hipMalloc(&b->btis, sizeof(int32_t) * cap) != hipSuccess ||
+        hipMalloc(&b->bstart, sizeof(int32_t) * cap) != hipSuccess) {
+      set_error("fw_batch_set_blocks: allocation failed");
+      b->cap_blocks = 0;
+      return FW_E_NOMEM;
+    }
+    b->cap_blocks = cap;
+    if (b->graph) {
+      (void)hipGraphExecDestroy(b->graph);
+      b->graph = nullptr;
+    }
+  }
+  if (nblocks &&
+      (hipMemcpyAsync(b->btis, btis, sizeof(int32_t) * nblocks, hipMemcpyHostToDevice, s) !=
+           hipSuccess ||
+       hipMemcpyAsync(b->bstart, bstart, sizeof(int32_t) * nblocks, hipMemcpyHostToDevice, s) !=
+           hipSuccess ||
+       hipStreamSynchronize(s) != hipSuccess)) {
+    set_error("fw_batch_set_blocks: copy failed");
+    return FW_E_HIP;
+  }
+  b->nblocks = nblocks;
+  return FW_OK;
+}
+
+int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph, void* stream) {
+  if (!b || n_steps < 0) {
+    set_error("fw_batch_run: bad arguments");
+    return FW_E_INVALID;
+  }
+  if (b->nblocks == 0 || n_steps == 0) return FW_OK;
+  BatchLauncher L = batch_launcher(b->precision, b->model);
+  hipStream_t s = as_stream(stream);
+  int32_t done = 0;
+  if (use_graph && n_steps >= 2) {
+    if (!b->graph || b->graph_nblocks != b->nblocks) {
+      if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+      }
+      hipStream_t cap;
+      if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+        set_error("fw_batch_run: stream creation failed");
+        return FW_E_HIP;
+      }
+      hipGraph_t g = nullptr;
+      int rc = FW_OK;
+      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+      if (e == hipSuccess) {
+        rc = L(batch_args(b, 0, b->step_words, b->step_words + 1), cap);
+        if (!rc) rc = L(batch_args(b, 0, b->step_words + 1, b->step_words), cap);
+        e = hipStreamEndCapture(cap, &g);
+      }
+      hipGraphExec_t ex = nullptr;
+      if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipStreamDestroy(cap);
+      if (e != hipSuccess || rc || !ex) {
+        set_error("fw_batch_run: capture failed (%s)", hipGetErrorString(e));
+        return rc ? rc : FW_E_HIP;
+      }
+      b->graph = ex;
+      b->graph_nblocks = b->nblocks;
+    }
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, b->step_words, (int32_t)step0);
+    const int32_t pairs = n_steps / 2;
+    for (int32_t q = 0; q < pairs; ++q) {
+      if (hipGraphLaunch(b->graph, s) != hipSuccess) {
+        set_error("fw_batch_run: graph launch failed");
+        return FW_E_HIP;
+      }
+    }
+    done = 2 * pairs;
+  }
+  for (int32_t q = done; q < n_steps; ++q) {
+    int rc = L(batch_args(b, (int32_t)(step0 + q), nullptr, nullptr), s);
+    if (rc) return rc;
+  }
+  return FW_OK;
+}
+
+int fw_batch_stim(fw_batch* b, int32_t slot, int64_t step, const fw_stim_desc* stim,
+                  void* stream) {
+  if (!b || !stim || slot < 0 || slot >= b->max_tissues || (stim->kind != 0 && stim->kind != 1)) {
+    set_error("fw_batch_stim: bad arguments");
+    return FW_E_INVALID;
+  }
+  const fw_tissue_desc& d = b->host[slot];
+  if (!d.u) {
+    set_error("fw_batch_stim: slot %d is empty", slot);
+    return FW_E_INVALID;
+  }
+  const int64_t ls = step - d.step0;
+  void* u = (ls & 1) ? d.un : d.u;
+  const int64_t shape[3] = {d.ni, d.nj, 1};
+  return launch_stim(b->precision, u, d.code, stim, shape, as_stream(stream));
+}
+
+}  // extern "C"

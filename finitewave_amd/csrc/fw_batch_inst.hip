@@ -1,0 +1,50 @@
+// fw_batch_inst.hip -- instantiates the batched step for one (precision,
+// model) pair, selected by FW_PREC and FW_MODEL like fw_step_inst.hip.
+#include <type_traits>
+
+#include "fw_batch.hpp"
+#include "fw_internal.hpp"
+
+#ifndef FW_PREC
+#error "FW_PREC must be defined"
+#endif
+#ifndef FW_MODEL
+#error "FW_MODEL must be defined"
+#endif
+
+#define FW_CAT4_(a, b, c, d) a##b##c##d
+#define FW_CAT4(a, b, c, d) FW_CAT4_(a, b, c, d)
+#define FW_BLAUNCHER FW_CAT4(fw_batch_launch_p, FW_PREC, _m, FW_MODEL)
+
+namespace fw {
+
+using BReal = std::conditional_t<FW_PREC == 0, double, float>;
+
+int FW_BLAUNCHER(const BatchLaunch& L, hipStream_t s) {
+  BatchArgs<BReal> a;
+  a.T = static_cast<const TissueDesc*>(L.table);
+  a.btis = L.btis;
+  a.bstart = L.bstart;
+  a.nblocks = L.nblocks;
+  a.per_xcd = L.per_xcd;
+  a.step_in = L.step_in;
+  a.step_out = L.step_out;
+  a.step = L.step;
+  a.D = L.D;
+  a.dtd = L.dt;
+  a.dr2 = L.dr2;
+  a.inv_dr2 = L.inv_dr2;
+  a.dt = (BReal)L.dt;
+  a.s = (BReal)(L.D * L.dt / L.dr2);
+  a.P = L.P;
+  hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal>), dim3(L.per_xcd * kXcds), dim3(kBlock), 0,
+                     s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("batch launch failed: %s", hipGetErrorString(e));
+    return FW_E_HIP;
+  }
+  return FW_OK;
+}
+
+}  // namespace fw

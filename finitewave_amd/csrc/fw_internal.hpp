@@ -65,3 +65,36 @@ int launch_act(int precision, int32_t* act, const void* u, int64_t n,
 void set_error(const char* fmt, ...);
 
 }  // namespace fw
+
+namespace fw {
+
+struct BatchLaunch {
+  const void* table;
+  const int32_t* btis;
+  const int32_t* bstart;
+  uint32_t nblocks, per_xcd;
+  const int32_t* step_in;
+  int32_t* step_out;
+  int32_t step;
+  double D, dt, dr2, inv_dr2;
+  Params P;
+};
+
+typedef int (*BatchLauncher)(const BatchLaunch& L, hipStream_t stream);
+BatchLauncher batch_launcher(int precision, int model);
+
+}  // namespace fw
+
+struct fw_batch {
+  int32_t model, precision, max_tissues;
+  double D, dt, dr2, inv_dr2;
+  fw::Params P;
+  fw_tissue_desc* table;  // device
+  int32_t* btis;          // device block table
+  int32_t* bstart;
+  int32_t nblocks, cap_blocks;
+  int32_t* step_words;
+  hipGraphExec_t graph;
+  int32_t graph_nblocks;
+  fw_tissue_desc* host;   // host mirror of the table
+};

@@ -1,0 +1,116 @@
+"""Serving: a request loop with continuous batching on one GPU.
+
+Requests arrive over wall-clock time.  A request is admitted when it has
+arrived, a batch slot is free and its HBM footprint fits the budget; it then
+joins the running batch of its model at the next launch boundary, so work
+admitted later overlaps on the device with work already running.  Each model
+has one ``TissueBatch``; the loop advances the batches in turn by short
+chunks of steps, retires finished tissues (their activation map, probe trace
+and APD90 are read back) and records each request's latency from arrival to
+results.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .batch import BatchSim, TissueBatch
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float          # seconds after the start of serving
+    model: str              # key of the server's batches
+    sim: BatchSim
+    footprint: int = 0
+    admitted: float | None = None
+    finished: float | None = None
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+class Server:
+    def __init__(self, model_classes: dict, precision: str = "fp32", dt: float = 0.01,
+                 dr: float = 0.25, hbm_budget: int | None = None, max_tissues: int = 256,
+                 chunk_steps: int = 200, use_graph: bool = True):
+        import torch
+        self.batches = {k: TissueBatch(cls, precision=precision, dt=dt, dr=dr,
+                                       max_tissues=max_tissues, use_graph=use_graph,
+                                       chunk_cap=chunk_steps)
+                        for k, cls in model_classes.items()}
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info()
+            hbm_budget = int(0.6 * free)
+        self.hbm_budget = hbm_budget
+        self.in_use = 0
+        self.chunk_steps = chunk_steps
+        self.completed: list[Request] = []
+        self.peak_in_use = 0
+        self.max_concurrent = 0
+
+    def footprint(self, req: Request) -> int:
+        b = self.batches[req.model]
+        steps = int(np.ceil(req.sim.t_max / b.dt))
+        return b.footprint(np.shape(req.sim.tissue.mesh), steps)
+
+    def serve(self, requests: list, realtime: bool = True, log=None) -> dict:
+        """Serve `requests` (sorted by arrival).  With realtime the arrival
+        times are honoured against the wall clock; otherwise every request is
+        available at once."""
+        import torch
+        pending = deque(sorted(requests, key=lambda r: r.arrival))
+        for r in pending:
+            r.footprint = self.footprint(r)
+            if r.footprint > self.hbm_budget:
+                raise ValueError(f"request {r.rid} needs {r.footprint} B, above the budget")
+        active: dict = {}
+        t0 = time.perf_counter()
+        while pending or active:
+            now = time.perf_counter() - t0
+            # admission
+            while pending and (not realtime or pending[0].arrival <= now):
+                r = pending[0]
+                b = self.batches[r.model]
+                if self.in_use + r.footprint > self.hbm_budget or not b.free:
+                    break
+                pending.popleft()
+                b.admit(r.sim)
+                r.admitted = time.perf_counter() - t0
+                active[id(r.sim)] = r
+                self.in_use += r.footprint
+                self.peak_in_use = max(self.peak_in_use, self.in_use)
+            self.max_concurrent = max(self.max_concurrent, len(active))
+            if not active:
+                if pending and realtime:
+                    time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
+                continue
+            # one chunk on every batch with work
+            for b in self.batches.values():
+                if not b.sims:
+                    continue
+                for sim in b.advance(self.chunk_steps):
+                    r = active.pop(id(sim))
+                    r.finished = time.perf_counter() - t0
+                    self.in_use -= r.footprint
+                    self.completed.append(r)
+                    if log:
+                        log(r)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        lat = np.array([r.latency for r in self.completed])
+        first = min(r.arrival for r in self.completed)
+        last = max(r.finished for r in self.completed)
+        return dict(completed=len(self.completed), wall_s=wall,
+                    sims_per_s=len(self.completed) / max(1e-9, last - first),
+                    p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
+                    peak_hbm_bytes=self.peak_in_use, max_concurrent=self.max_concurrent)
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()

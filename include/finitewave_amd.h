@@ -4,8 +4,10 @@
  * Plain C: pointers, sizes, status codes.  Every array argument is a device
  * pointer (HIP), C-order over the grid shape, unless stated otherwise.  No
  * entry point allocates device memory except fw_plan_create (a small
- * parameter block).  `stream` is a hipStream_t passed as void* (NULL = the
- * default stream).  All launches are asynchronous on that stream.
+ * parameter block), fw_batch_create (the tissue descriptor table) and
+ * fw_batch_set_blocks (the block table, when it grows).  `stream` is a
+ * hipStream_t passed as void* (NULL = the default stream).  All launches are
+ * asynchronous on that stream.
  *
  * Status: 0 on success, negative on error; fw_last_error() returns a
  * thread-local message for the last failure on the calling thread.
@@ -35,6 +37,9 @@
  *                          cpuwave3D/stimulation/*.py)
  *   fw_act_time_update  <- ActivationTime2DTracker._track
  *                          (cpuwave2D/tracker/activation_time_2d_tracker.py:61-63)
+ *   fw_batch_*          <- no counterpart (the reference steps one tissue per
+ *                          process); many CardiacModel2D.run() calls of one
+ *                          model sharing each step's launch.
  */
 #ifndef FINITEWAVE_AMD_H
 #define FINITEWAVE_AMD_H
@@ -202,6 +207,57 @@ int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim,
 /* One activation-time pass over all N nodes (act = step where u > thr first) */
 int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr,
                        int32_t step, void* stream);
+
+/* ---- batches: many 2D tissues of one model, one launch per step ----------
+ * No counterpart in the reference, which runs one tissue per process; this
+ * is the ensemble / serving path built on the same fused step.  Each tissue
+ * owns its device buffers and is described by one fw_tissue_desc; a block
+ * table maps the launch's 256-node blocks to tissues; fw_batch_create and
+ * fw_batch_set_blocks allocate device memory.  Local step of a
+ * tissue = global step - step0; its u is in `u` at even local steps and in
+ * `un` at odd ones.  act records local steps, probe[ls] = u at probe_idx
+ * before local step ls, in the batch's precision (double or float). */
+typedef struct fw_tissue_desc {
+  void* u;
+  void* un;
+  void* state[FW_MAX_STATE];
+  const uint8_t* code;
+  int32_t* act;
+  void* probe;
+  int64_t ni, nj, n;
+  double d[4]; /* uniform half-step fields: iso xx, yy; aniso xx, xy, yx, yy */
+  double act_thr;
+  int32_t stencil; /* FW_STENCIL_ISO or FW_STENCIL_ANISO */
+  int32_t step0;
+  int32_t act_start; /* first local step recorded in act, < 0 never */
+  int32_t probe_idx; /* local flat node, < 0 none */
+  int32_t probe_len; /* entries of probe; local steps beyond it are not sampled */
+  int32_t reserved;
+} fw_tissue_desc;
+
+typedef struct fw_batch fw_batch;
+
+/* Allocates the descriptor table for max_tissues slots. */
+int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
+                    double dr2, const double* params, int32_t n_params,
+                    int32_t max_tissues, fw_batch** out);
+int fw_batch_destroy(fw_batch* batch);
+/* Writes slot's descriptor (host struct copied to the device table). */
+int fw_batch_set_tissue(fw_batch* batch, int32_t slot, const fw_tissue_desc* desc,
+                        void* stream);
+/* Replaces the block table (host arrays of nblocks entries: tissue slot and
+ * first local node of each 256-node block); grows its device buffer when
+ * needed. */
+int fw_batch_set_blocks(fw_batch* batch, const int32_t* block_tissue,
+                        const int32_t* block_start, int32_t nblocks, void* stream);
+/* n_steps global steps from step0; use_graph replays a captured two-step
+ * graph (recaptured when the block count changes). */
+int fw_batch_run(fw_batch* batch, int64_t step0, int32_t n_steps, int32_t use_graph,
+                 void* stream);
+/* Applies a stimulus to slot's potential as it stands before global step
+ * `step` (box in the tissue's own coordinates, third axis ignored). */
+int fw_batch_stim(fw_batch* batch, int32_t slot, int64_t step, const fw_stim_desc* stim,
+                  void* stream);
 
 #ifdef __cplusplus
 }

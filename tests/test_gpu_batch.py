@@ -1,0 +1,186 @@
+"""Batched stepping and the serving loop.
+
+Every tissue of a batch must give the same activation map, probe trace and
+final potential as ``model.run()`` on that tissue alone (the single-tissue
+path that is itself checked against the oracle), bit for bit, whatever else
+shares its launches: tissues of other sizes, fibrosis and stimulus
+protocols, admitted at other steps."""
+from __future__ import annotations
+
+import copy
+
+import numpy as np
+import pytest
+
+import finitewave_amd as fw
+from finitewave_amd.batch import BatchSim, TissueBatch
+from finitewave_amd.serving import Request, Server
+from finitewave_amd.workloads import fibrotic_tissue, restitution_ensemble
+
+
+def _single(cls, precision, sim: BatchSim, dt=0.01, dr=0.25):
+    model = cls()
+    model.precision = precision
+    model.dt, model.dr, model.t_max = dt, dr, sim.t_max
+    model.prog_bar = False
+    model.cardiac_tissue = copy.deepcopy(sim.tissue)
+    seq = fw.StimSequence()
+    for s in sim.stims:
+        seq.add_stim(copy.deepcopy(s))
+    model.stim_sequence = seq
+    ts = fw.TrackerSequence()
+    act = fw.ActivationTime2DTracker()
+    act.threshold = sim.act_threshold
+    act.start_time = sim.act_start
+    ts.add_tracker(act)
+    ap = fw.ActionPotential2DTracker()
+    ap.cell_ind = list(sim.probe)
+    ap.step = 1
+    ts.add_tracker(ap)
+    model.tracker_sequence = ts
+    model.run()
+    return dict(u=np.array(model.u), act_t=np.array(act.output), trace=np.array(ap.output))
+
+
+def _sims(kind, n=3, t_max=8.0, seed=0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        side = int(rng.integers(20, 60))
+        fib = None if kind == "iso" else float(rng.uniform(0, 180))
+        tissue = fibrotic_tissue(side, float(rng.uniform(0, 0.25)), 100 + k, fibre_deg=fib)
+        n_ = side + 2
+        if kind == "tp06":
+            thr = -40.0
+            stims = [fw.StimCurrentCoord2D(0, 100, 1.0, 0, n_, 0, 5)]
+            if k % 2:
+                stims.append(fw.StimCurrentCoord2D(4.0, 100, 1.0, n_ // 2, n_, 0, n_ // 2))
+        else:
+            thr = 0.5
+            stims = [fw.StimVoltageCoord2D(0, 1.0, 0, n_, 0, 5)]
+            if k % 2:
+                stims.append(fw.StimVoltageCoord2D(3.0, 1.0, n_ // 2, n_, 0, n_ // 2))
+        mid = n_ // 2
+        probe = (mid, int(np.flatnonzero(tissue.mesh[mid] == 1)[0]))
+        out.append(BatchSim(tissue=tissue, t_max=t_max + 0.37 * k, stims=stims, probe=probe,
+                            act_threshold=thr, act_start=0.5 * k, apd_threshold=thr))
+    return out
+
+
+CASES = [
+    ("aliev_panfilov", "fp64", "iso"), ("aliev_panfilov", "fp64", "aniso"),
+    ("tp06", "fp64", "tp06"), ("tp06", "fp32", "tp06"),
+    ("mitchell_schaeffer", "fp32", "iso"), ("mitchell_schaeffer", "fp32", "aniso"),
+]
+CLS = {"aliev_panfilov": fw.AlievPanfilov2D, "tp06": fw.TP062D,
+       "mitchell_schaeffer": fw.MitchellSchaeffer2D}
+
+
+def _check_same(res, ref):
+    assert np.array_equal(res["act_t"], ref["act_t"])
+    assert np.array_equal(res["trace"], ref["trace"].astype(np.float64))
+    assert np.array_equal(res["u"], ref["u"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,prec,kind", CASES, ids=["-".join(c) for c in CASES])
+def test_batch_equals_single_tissue_run(gpu, model, prec, kind):
+    cls = CLS[model]
+    sims = _sims(kind)
+    batch = TissueBatch(cls, precision=prec, max_tissues=8, chunk_cap=97)
+    # staggered admission: the second and third tissues join a running batch
+    batch.admit(sims[0])
+    batch.advance(113)
+    batch.admit(sims[1])
+    batch.advance(58)
+    batch.admit(sims[2])
+    done = batch.run_all()
+    batch.close()
+    assert len(done) == 3
+    for sim in sims:
+        ref = _single(cls, prec, sim)
+        _check_same(sim.result, ref)
+
+
+@pytest.mark.gpu
+def test_batch_graph_equals_plain_launches(gpu):
+    outs = []
+    for graph in (True, False):
+        sims = _sims("iso", seed=3)
+        b = TissueBatch(fw.AlievPanfilov2D, precision="fp32", use_graph=graph, chunk_cap=64)
+        for s in sims:
+            b.admit(s)
+        b.run_all()
+        b.close()
+        outs.append([s.result for s in sims])
+    for a, b in zip(*outs):
+        _check_same(a, b)
+
+
+@pytest.mark.gpu
+def test_serving_result_alone_equals_in_mix(gpu):
+    """Each request's result is the same served alone and served among a mix
+    of other requests (two models, staggered arrivals, slot reuse)."""
+    def reqs():
+        out = []
+        for k, s in enumerate(_sims("tp06", n=3, t_max=6.0, seed=5)):
+            out.append(Request(rid=k, arrival=0.0, model="tp06", sim=s))
+        for k, s in enumerate(_sims("aniso", n=4, t_max=7.0, seed=6)):
+            out.append(Request(rid=10 + k, arrival=0.0, model="ap", sim=s))
+        return out
+    classes = {"tp06": fw.TP062D, "ap": fw.AlievPanfilov2D}
+    mix = reqs()
+    srv = Server(classes, precision="fp32", max_tissues=2, chunk_steps=150,
+                 hbm_budget=1 << 30)
+    stats = srv.serve(mix, realtime=False)
+    srv.close()
+    assert stats["completed"] == len(mix)
+    assert stats["max_concurrent"] <= 4
+    for r in reqs():
+        alone = Server(classes, precision="fp32", max_tissues=2, chunk_steps=150,
+                       hbm_budget=1 << 30)
+        alone.serve([r], realtime=False)
+        alone.close()
+        other = [m for m in mix if m.rid == r.rid][0]
+        _check_same(r.sim.result, other.sim.result)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fit", [1, 2, 3])
+def test_serving_admission_respects_hbm_budget(gpu, fit):
+    """Five equal requests under a budget that holds `fit` of them."""
+    classes = {"ap": fw.AlievPanfilov2D}
+    srv = Server(classes, precision="fp32", max_tissues=8, chunk_steps=50)
+    reqs = []
+    for k in range(5):
+        t = fibrotic_tissue(30, 0.05, 200 + k)
+        sim = BatchSim(tissue=t, t_max=2.0, stims=[fw.StimVoltageCoord2D(0, 1.0, 0, 32, 0, 4)],
+                       probe=(16, 16), act_threshold=0.5)
+        reqs.append(Request(rid=k, arrival=0.0, model="ap", sim=sim))
+    need = srv.footprint(reqs[0])
+    srv.hbm_budget = (fit + 1) * need - 1
+    stats = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert stats["completed"] == 5
+    assert stats["peak_hbm_bytes"] <= srv.hbm_budget
+    assert stats["max_concurrent"] == fit
+
+
+@pytest.mark.gpu
+def test_restitution_ensemble_sample(gpu):
+    """A few tissues of the restitution ensemble (MS, S1 train + S2): every
+    tissue is captured by the S2 beat, APD90 grows with the coupling
+    interval, and the S2 activation map of a tissue equals its single run."""
+    cls, sims = restitution_ensemble(n=6, sides=(32,), beats=2, bcl=300.0, tail=300.0,
+                                     p_range=(0.0, 0.1), ci_range=(330.0, 450.0))
+    sims = sorted(sims, key=lambda s: s.tag["ci"])
+    b = TissueBatch(cls, precision="fp32", max_tissues=8)
+    for s in sims:
+        b.admit(s)
+    b.run_all()
+    b.close()
+    apds = [s.result["apd90"] for s in sims]
+    assert all(a is not None for a in apds)
+    assert apds[-1] > apds[0]
+    ref = _single(cls, "fp32", sims[0])
+    _check_same(sims[0].result, ref)

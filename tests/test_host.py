@@ -159,3 +159,121 @@ def test_model_classes_keep_reference_attributes():
     t = fw.CardiacTissue2D([4, 4])
     t.fibers = np.zeros((4, 4, 2))
     assert isinstance(fw.Barkley2D().select_stencil(t), fw.AsymmetricStencil2D)
+
+
+# ---------------------------------------------------------------- batches (host side)
+from finitewave_amd.analysis import apd90, conduction_velocity  # noqa: E402
+from finitewave_amd.batch import (BatchSim, _uniform_fields, run_length,  # noqa: E402
+                                  stim_schedule, time_fold)
+
+
+@pytest.mark.parametrize("angle", [0.0, 30.0, 97.5])
+def test_batch_uniform_fields_equal_stencil_fields_bitwise(angle):
+    n = 12
+    t = fw.CardiacTissue2D([n, n])
+    t.conductivity = 0.7
+    a = np.deg2rad(angle)
+    f = np.zeros((n, n, 2))
+    f[..., 0], f[..., 1] = np.cos(a), np.sin(a)
+    t.fibers = f
+    fields = fw.AsymmetricStencil2D().half_step_fields(None, t)
+    d4 = _uniform_fields(BatchSim(tissue=t, t_max=1.0), _native.STENCIL_ANISO)
+    for k in range(4):
+        assert np.all(fields[k] == d4[k]) and np.all(np.signbit(fields[k]) == np.signbit(d4[k]))
+    t.fibers = None
+    iso = fw.IsotropicStencil2D().half_step_fields(None, t)
+    d2 = _uniform_fields(BatchSim(tissue=t, t_max=1.0), _native.STENCIL_ISO)
+    assert np.all(iso[0] == d2[0]) and np.all(iso[1] == d2[1])
+
+
+def test_batch_rejects_non_uniform_fields():
+    t = fw.CardiacTissue2D([8, 8])
+    t.fibers = random_fibers((8, 8))
+    with pytest.raises(ValueError):
+        _uniform_fields(BatchSim(tissue=t, t_max=1.0), _native.STENCIL_ANISO)
+    t.fibers = None
+    t.conductivity = np.ones((8, 8))
+    with pytest.raises(ValueError):
+        _uniform_fields(BatchSim(tissue=t, t_max=1.0), _native.STENCIL_ISO)
+
+
+def _loop_schedule(stims, dt, n_steps):
+    """Stimulus applications of the model loop (stim_sequence.py:74-77):
+    the stimuli's own update_status on a stand-in model."""
+    class M:
+        t = 0.0
+    m = M()
+    out = []
+    for k in range(n_steps):
+        for s in stims:
+            if m.t >= s.t and not s.passed:
+                out.append((k, s))
+                s.update_status(m)
+        m.t += dt
+    return out
+
+
+def test_batch_stim_schedule_follows_the_model_loop():
+    dt = 0.01
+    stims = [fw.StimCurrentCoord2D(0, 5, 0.5, 0, 5, 0, 3),
+             fw.StimVoltageCoord2D(0.3, 1.0, 0, 4, 0, 4),
+             fw.StimCurrentCoord2D(1.234, 2, 0.07, 0, 5, 0, 3),
+             fw.StimVoltageCoord2D(9.0, 1.0, 0, 4, 0, 4),
+             fw.StimCurrentCoord2D(2.0, 1, 0.0, 0, 5, 0, 3)]
+    n = 700
+    fold = time_fold(dt, n + 1)
+    got = stim_schedule(stims, n, fold)
+    for s in stims:
+        s.passed = False
+    ref = _loop_schedule(stims, dt, n)
+    assert [(k, id(s)) for k, s in got] == [(k, id(s)) for k, s in ref]
+    # a 0.5 ms current at dt = 0.01 applies 51 times (the reference's fencepost)
+    assert sum(1 for _, s in got if s is stims[0]) == 51
+    assert not any(s is stims[3] for _, s in got)
+
+
+@pytest.mark.parametrize("t_max", [1.0, 8.37, 10.0, 3.3333, 0.005])
+def test_batch_run_length_equals_model_loop(t_max):
+    dt = 0.01
+    fold = time_fold(dt, int(np.ceil(t_max / dt)) + 2)
+    # the model loop: iters steps, stopping when t > t_max (cardiac_model.py:191-203)
+    t, step = 0.0, 0
+    for _ in range(int(np.ceil(t_max / dt))):
+        t += dt
+        step += 1
+        if t > t_max or step > int(np.ceil(t_max / dt)):
+            break
+    assert run_length(t_max, dt, fold) == step
+
+
+def test_time_fold_is_the_left_fold():
+    f = time_fold(0.01, 1001)
+    assert f[1000] == 9.999999999999831
+    assert f[0] == 0.0 and f[1] == 0.01
+
+
+def test_apd90_closed_form_on_a_synthetic_trace():
+    dt = 0.1
+    t = np.arange(0, 600, dt)
+    # rest 0, linear upstroke over 100..102 to 1, linear repolarisation to 0 by 402
+    u = np.where(t < 100, 0.0, np.minimum((t - 100) / 2, 1 - (t - 102) / 300))
+    u = np.clip(u, 0, 1)
+    apd = apd90(u, dt, 0.5)
+    # from the last sample below 0.5 (t = 100.9) to the first below 0.1 (t ~ 372)
+    assert abs(apd - (372.0 - 100.9)) <= 2 * dt
+    # the last of two beats is measured
+    u2 = np.concatenate([u, 0.5 * u])
+    assert abs(apd90(u2, dt, 0.25) - (372.0 - 100.9)) <= 2 * dt
+    assert apd90(np.zeros(100), dt, 0.5) is None
+    # the tutorial's quirk is kept: a jump from below the 90% level straight
+    # over the threshold gives 0
+    jump = np.where(t < 100, 0.0, np.clip(1 - (t - 100) / 300, 0, 1))
+    assert apd90(jump, dt, 0.5) == 0.0
+
+
+def test_conduction_velocity_closed_form():
+    dr = 0.25
+    act = np.tile(np.arange(41) * 0.5, (3, 1))      # 0.5 ms per node
+    assert abs(conduction_velocity(act, 1, dr) - dr / 0.5) < 1e-12
+    act[1, 30] = -1
+    assert conduction_velocity(act, 1, dr) is None
