@@ -148,6 +148,146 @@ def cpu_baseline(steps: int, threads: int):
                        f"{dt:.1f} s on {used} threads")
 
 
+def ensemble_leg(n_tissues: int, steps: int, full: bool):
+    """The restitution ensemble (Mitchell-Schaeffer, FP32): every tissue is
+    admitted into one batch, the S1 beat is applied, and `steps` steps of
+    the whole batch are timed (all tissues are present for the whole S1
+    train, so the per-step cost is stationary).  With `full` the ensemble is
+    also run to completion and its sims/s measured."""
+    import torch
+    from finitewave_amd.batch import TissueBatch
+    from finitewave_amd.workloads import restitution_ensemble
+    cls, sims = restitution_ensemble(n=n_tissues)
+    b = TissueBatch(cls, precision="fp32", max_tissues=n_tissues, chunk_cap=4000)
+    t_adm = time.perf_counter()
+    for s in sims:
+        b.admit(s)
+    torch.cuda.synchronize()
+    t_adm = time.perf_counter() - t_adm
+    b.advance(200)  # S1 at step 0, graph capture, warm-up
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    done = 0
+    while done < steps:
+        n0 = b.step
+        b.advance(steps - done)
+        done += b.step - n0
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    n_active = sum(int(np.count_nonzero(s.tissue.mesh == 1)) for s in sims)
+    n_nodes = sum(int(s.tissue.mesh.size) for s in sims)
+    sec = ms / 1e3 / done
+    # MS, FP32: u, u_new and h (read + write) per active node, the node code per node
+    bytes_step = 16 * n_active + n_nodes
+    total_cell_steps = sum(int(np.count_nonzero(s.tissue.mesh == 1)) *
+                           int(math.ceil(s.t_max / 0.01)) for s in sims)
+    out = dict(workload=f"{n_tissues} Mitchell-Schaeffer 2D tissues, side U{{64,128,256,512}}+2, "
+                        "fibrosis p~U[0,0.35], S1 x5 @ 400 ms + S2 @ CI~U[150,400] ms, "
+                        "isotropic, dt 0.01, dr 0.25, FP32",
+               tissues=n_tissues, nodes=n_nodes, active=n_active, steps_timed=done,
+               ms_per_step=round(sec * 1e3, 5), gcell=round(n_active / sec / 1e9, 3),
+               admit_s=round(t_adm, 3),
+               roofline={"bound": "hbm", "achieved": round(bytes_step / sec / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(bytes_step / sec / 1e9 / HBM_PEAK_GBS, 4),
+                         "bytes_per_step": bytes_step},
+               projected_s=round(total_cell_steps / (n_active / sec), 1))
+    if full:
+        w0 = time.perf_counter()
+        b.run_all()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0 + done * sec
+        res = [s.result for s in sims]
+        out["full"] = dict(wall_s=round(wall, 2), sims_per_s=round(n_tissues / wall, 3),
+                           apd90_found=sum(r.get("apd90") is not None for r in res),
+                           cv_found=sum(r.get("cv") is not None for r in res))
+    b.close()
+    return out
+
+
+def _fp64_plain(cls, sim):
+    """One request on the plain FP64 single-tissue path (model.run())."""
+    import copy
+    import finitewave_amd as fw
+    model = cls()
+    model.precision = "fp64"
+    model.dt, model.dr, model.t_max = 0.01, 0.25, sim.t_max
+    model.prog_bar = False
+    model.cardiac_tissue = copy.deepcopy(sim.tissue)
+    seq = fw.StimSequence()
+    for s in sim.stims:
+        seq.add_stim(copy.deepcopy(s))
+    model.stim_sequence = seq
+    ts = fw.TrackerSequence()
+    act = fw.ActivationTime2DTracker()
+    act.threshold = sim.act_threshold
+    ts.add_tracker(act)
+    ap = fw.ActionPotential2DTracker()
+    ap.cell_ind = list(sim.probe)
+    ap.step = 1
+    ts.add_tracker(ap)
+    model.tracker_sequence = ts
+    model.run()
+    return np.array(act.output), np.array(ap.output)
+
+
+def serving_leg(n_req: int, fp64_baseline: bool):
+    """The Poisson request stream served with continuous batching (FP32,
+    TP06 calcium in FP64), then, for comparison and for the error figures,
+    the same requests one after another on the plain FP64 path."""
+    import torch
+    from finitewave_amd.analysis import apd90
+    from finitewave_amd.serving import Request, Server
+    from finitewave_amd.workloads import serving_stream
+    classes, specs = serving_stream(n=n_req)
+    reqs = [Request(rid=k, arrival=s["arrival"], model=s["model"], sim=s["sim"])
+            for k, s in enumerate(specs)]
+    srv = Server(classes, precision="fp32", max_tissues=64, chunk_steps=1000)
+    stats = srv.serve(reqs, realtime=True)
+    srv.close()
+    cell_steps = sum(int(np.count_nonzero(r.sim.tissue.mesh == 1)) * r.sim.n_steps for r in reqs)
+    out = dict(workload=f"{n_req} requests, Poisson 200/s, TP06 or Aliev-Panfilov 2D, side "
+                        "U{128..1024}+2, fibrosis p~U[0,0.4], fibre U[0,180) deg (9-point), "
+                        "planar or S1-S2 cross-field, horizon U[500,2000] ms, FP32 (TP06 Ca "
+                        "FP64)",
+               requests=n_req, sims_per_s=round(stats["sims_per_s"], 4),
+               p50_s=round(stats["p50_s"], 3), p99_s=round(stats["p99_s"], 3),
+               wall_s=round(stats["wall_s"], 2),
+               gcell=round(cell_steps / stats["wall_s"] / 1e9, 3),
+               max_concurrent=stats["max_concurrent"],
+               peak_hbm_mb=round(stats["peak_hbm_bytes"] / 2**20, 1))
+    if fp64_baseline:
+        torch.cuda.synchronize()
+        t_plain = 0.0
+        lat_max, lat_bad, lat_n, apd_err = 0.0, 0, 0, []
+        for r in reqs:
+            w0 = time.perf_counter()
+            act64, trace64 = _fp64_plain(classes[r.model], r.sim)
+            torch.cuda.synchronize()
+            t_plain += time.perf_counter() - w0
+            a32 = r.sim.result["act_t"]
+            both = (a32 >= 0) & (act64 >= 0)
+            d = np.abs(a32[both] - act64[both])
+            lat_n += int(both.sum())
+            lat_bad += int(np.count_nonzero(d > 0.01 + 1e-9)) + \
+                int(np.count_nonzero((a32 >= 0) != (act64 >= 0)))
+            lat_max = max(lat_max, float(d.max()) if d.size else 0.0)
+            a = r.sim.result.get("apd90")
+            a64 = apd90(trace64, 0.01, r.sim.apd_threshold)
+            if a is not None and a64 is not None:
+                apd_err.append(abs(a - a64))
+        plain_rate = n_req / t_plain
+        out["fp64_plain"] = dict(sims_per_s=round(plain_rate, 4), wall_s=round(t_plain, 2))
+        out["speedup_vs_fp64_plain"] = round(stats["sims_per_s"] / plain_rate, 2)
+        out["errors_vs_fp64"] = dict(lat_max_ms=lat_max, lat_nodes=lat_n,
+                                     lat_over_1dt=lat_bad,
+                                     apd90_max_ms=max(apd_err) if apd_err else None,
+                                     apd90_compared=len(apd_err))
+    return out
+
+
 def copy_calibration(mbytes: int = 1024) -> None:
     """A plain device copy of known size, for scaling the HBM counters."""
     import torch
@@ -171,7 +311,15 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp64", "fp32"], default=None)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--ensemble-tissues", type=int, default=512)
+    ap.add_argument("--ensemble-steps", type=int, default=2000)
+    ap.add_argument("--ensemble-full", action="store_true",
+                    help="also run the whole ensemble to completion")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--no-fp64-serving", action="store_true")
+    ap.add_argument("--legs", default="stencil,ensemble,serving")
     args = ap.parse_args(argv)
+    legs = set(args.legs.split(","))
     if args.gpus != 1:
         print("bench.py measures one GPU: every MI355X box here has one device and the "
               "stepping path has no multi-GPU mode (slab decomposition is a design note); "
@@ -229,6 +377,13 @@ def main(argv=None) -> int:
         line["fp32_mixed"] = {"value": round(f["gcell"], 4), "unit": "Gcell-steps/s",
                               "ms_per_step": round(f["ms_per_step"], 5),
                               "roofline": f["roofline"]}
+    if "ensemble" in legs:
+        line["ensemble"] = ensemble_leg(args.ensemble_tissues, args.ensemble_steps,
+                                        args.ensemble_full)
+        print(f"# ensemble: {line['ensemble']}", file=sys.stderr, flush=True)
+    if "serving" in legs:
+        line["serving"] = serving_leg(args.serve_requests, not args.no_fp64_serving)
+        print(f"# serving: {line['serving']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps, threads)

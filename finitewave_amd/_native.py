@@ -29,7 +29,9 @@ EXPORTS = (
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
     "fw_stim_apply", "fw_act_time_update", "fw_batch_create", "fw_batch_destroy",
-    "fw_batch_set_tissue", "fw_batch_set_blocks", "fw_batch_run", "fw_batch_stim",
+    "fw_batch_set_tissue", "fw_batch_set_active", "fw_batch_release", "fw_batch_block_nodes",
+    "fw_batch_run",
+    "fw_batch_stim",
 )
 
 

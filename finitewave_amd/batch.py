@@ -140,8 +140,9 @@ class TissueBatch:
         L.fw_batch_destroy.argtypes = [ctypes.c_void_p]
         L.fw_batch_set_tissue.argtypes = [ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.POINTER(_TissueDesc), ctypes.c_void_p]
-        L.fw_batch_set_blocks.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                          ctypes.c_int32, ctypes.c_void_p]
+        L.fw_batch_set_active.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                          ctypes.c_void_p]
+        L.fw_batch_release.argtypes = [ctypes.c_void_p, ctypes.c_int32]
         L.fw_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_void_p]
         L.fw_batch_stim.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
@@ -185,6 +186,8 @@ class TissueBatch:
         if mesh.ndim != 2:
             raise ValueError("batches hold 2D tissues")
         tissue.check_boundary()
+        if tissue.special_boundaries is not None and np.any(tissue.special_boundaries):
+            raise ValueError("batched tissues take no special boundaries")
         shape = mesh.shape
         n = int(mesh.size)
         slot = self.free.pop()
@@ -260,6 +263,7 @@ class TissueBatch:
             if sim.apd_threshold is not None:
                 res["apd90"] = apd90(trace, self.dt, sim.apd_threshold)
         sim.result = res
+        check(lib().fw_batch_release(self.handle, slot), "fw_batch_release")
         self.free.append(slot)
         self.blocks_dirty = True
         return sim
@@ -269,16 +273,9 @@ class TissueBatch:
         return self.torch.cuda.current_stream().cuda_stream
 
     def _rebuild_blocks(self) -> None:
-        bt, bs = [], []
-        for slot in sorted(self.sims):
-            n = self.buffers[slot]["desc"].n
-            nb = (n + 255) // 256
-            bt.append(np.full(nb, slot, dtype=np.int32))
-            bs.append(np.arange(nb, dtype=np.int32) * 256)
-        btis = np.concatenate(bt) if bt else np.zeros(0, np.int32)
-        bstart = np.concatenate(bs) if bs else np.zeros(0, np.int32)
-        check(lib().fw_batch_set_blocks(self.handle, btis.ctypes.data, bstart.ctypes.data,
-                                        len(btis), self._stream()), "fw_batch_set_blocks")
+        slots = np.array(sorted(self.sims), dtype=np.int32)
+        check(lib().fw_batch_set_active(self.handle, slots.ctypes.data, len(slots),
+                                        self._stream()), "fw_batch_set_active")
         self.blocks_dirty = False
 
     def _apply_due_stims(self) -> None:

@@ -24,7 +24,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", str(CSRC)]
 FP64 = ["-ffp-contract=off"]
-FP32 = ["-ffp-contract=fast", "-fno-math-errno", "-freciprocal-math", "-fno-signed-zeros",
+FP32 = ["-ffp-contract=fast", "-fno-math-errno", "-freciprocal-math", "-fapprox-func",
+        "-fno-signed-zeros",
         "-fno-trapping-math"]
 
 

@@ -1,8 +1,10 @@
 // fw_abi.hip -- the extern "C" surface declared in include/finitewave_amd.h.
 #include <stdarg.h>
+#include <stdlib.h>
 #include <stdio.h>
 #include <string.h>
 
+#include "fw_batch.hpp"
 #include "fw_internal.hpp"
 
 namespace fw {
@@ -35,6 +37,25 @@ StepLauncher step_launcher(int precision, int model) {
 }
 
 __global__ void k_set_word(int32_t* p, int32_t v) { *p = v; }
+
+// Admission check of a batched tissue's codes: bit 0 of *bad when an updated
+// node (code & 3 == 1) lies on the outer layer, bit 1 when a node is frozen
+// (code bit 1).  Grid-stride over all n = ni * nj nodes.
+__global__ void k_check_codes(const uint8_t* __restrict__ code, int64_t ni, int64_t nj,
+                              int32_t* bad) {
+  const int64_t n = ni * nj;
+  int found = 0;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+       c += (int64_t)gridDim.x * blockDim.x) {
+    const uint8_t cd = code[c];
+    if (cd & 2) found |= 2;
+    if ((cd & 3) == 1) {
+      const int64_t i = c / nj, j = c - i * nj;
+      if (i == 0 || i == ni - 1 || j == 0 || j == nj - 1) found |= 1;
+    }
+  }
+  if (found) atomicOr(bad, found);
+}
 
 }  // namespace fw
 
@@ -466,14 +487,13 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   b->inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
   for (int q = 0; q < FW_MAX_PARAMS; ++q) b->P.v[q] = q < n_params ? params[q] : 0.0;
   b->host = new (std::nothrow) fw_tissue_desc[max_tissues]();
-  if (!b->host || hipMalloc(&b->table, sizeof(fw_tissue_desc) * max_tissues) != hipSuccess ||
-      hipMalloc(&b->step_words, 2 * sizeof(int32_t)) != hipSuccess) {
+  if (!b->host || hipMalloc(&b->table, sizeof(BatchTissue) * max_tissues) != hipSuccess ||
+      hipMalloc(&b->step_words, 4 * sizeof(int32_t)) != hipSuccess) {
     set_error("fw_batch_create: allocation failed");
-    delete[] b->host;
-    delete b;
+    (void)fw_batch_destroy(b);
     return FW_E_NOMEM;
   }
-  if (hipMemset(b->table, 0, sizeof(fw_tissue_desc) * max_tissues) != hipSuccess) {
+  if (hipMemset(b->table, 0, sizeof(BatchTissue) * max_tissues) != hipSuccess) {
     set_error("fw_batch_create: table init failed");
     (void)fw_batch_destroy(b);
     return FW_E_HIP;
@@ -499,8 +519,9 @@ int fw_batch_set_tissue(fw_batch* b, int32_t slot, const fw_tissue_desc* d, void
     set_error("fw_batch_set_tissue: bad arguments");
     return FW_E_INVALID;
   }
-  if (d->n != d->ni * d->nj || d->ni < 3 || d->nj < 3 || !d->u || !d->un || !d->code ||
-      (d->stencil != FW_STENCIL_ISO && d->stencil != FW_STENCIL_ANISO)) {
+  if (d->n != d->ni * d->nj || d->ni < 3 || d->nj < 3 || d->n > INT32_MAX / 4 || !d->u ||
+      !d->un || !d->code || (d->stencil != FW_STENCIL_ISO && d->stencil != FW_STENCIL_ANISO) ||
+      (d->probe && d->probe_idx >= d->n)) {
     set_error("fw_batch_set_tissue: malformed descriptor for slot %d", slot);
     return FW_E_INVALID;
   }
@@ -509,59 +530,153 @@ int fw_batch_set_tissue(fw_batch* b, int32_t slot, const fw_tissue_desc* d, void
       set_error("fw_batch_set_tissue: state %d of slot %d is null", k, slot);
       return FW_E_INVALID;
     }
-  b->host[slot] = *d;
   hipStream_t s = as_stream(stream);
-  if (hipMemcpyAsync(b->table + slot, b->host + slot, sizeof(fw_tissue_desc),
+  // no updated node on the outer layer (the kernel then needs no column
+  // guard) and no frozen node (non-updated nodes are written back unchanged)
+  int32_t bad = 0;
+  if (hipMemsetAsync(b->step_words + 2, 0, sizeof(int32_t), s) != hipSuccess) {
+    set_error("fw_batch_set_tissue: flag reset failed");
+    return FW_E_HIP;
+  }
+  const int64_t nblk = (d->n + 255) / 256;
+  hipLaunchKernelGGL(k_check_codes, dim3((unsigned)(nblk < 4096 ? nblk : 4096)), dim3(256), 0,
+                     s, d->code, d->ni, d->nj, b->step_words + 2);
+  if (hipMemcpyAsync(&bad, b->step_words + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s) !=
+          hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("fw_batch_set_tissue: border check failed");
+    return FW_E_HIP;
+  }
+  if (bad & 1) {
+    set_error("fw_batch_set_tissue: slot %d has updated nodes (code & 3 == 1) on its outer "
+              "layer", slot);
+    return FW_E_INVALID;
+  }
+  if (bad & 2) {
+    set_error("fw_batch_set_tissue: slot %d has frozen nodes (code bit 1); batched tissues "
+              "take no special boundaries", slot);
+    return FW_E_INVALID;
+  }
+  BatchTissue r;
+  memset(&r, 0, sizeof(r));
+  r.u = d->u;
+  r.un = d->un;
+  r.code = d->code;
+  r.act = d->act;
+  r.probe = d->probe;
+  for (int k = 0; k < FW_MAX_STATE; ++k) r.state[k] = d->state[k];
+  for (int q = 0; q < 4; ++q) {
+    r.dd[q] = d->d[q];
+    r.df[q] = (float)d->d[q];
+  }
+  r.act_thr = d->act_thr;
+  r.act_thr_f = (float)d->act_thr;
+  r.n = (int32_t)d->n;
+  r.nj = (int32_t)d->nj;
+  r.step0 = d->step0;
+  r.act_start = d->act_start;
+  r.probe_idx = d->probe ? d->probe_idx : -1;
+  r.probe_len = d->probe ? d->probe_len : 0;
+  r.stencil = d->stencil;
+  if (hipMemcpyAsync(static_cast<BatchTissue*>(b->table) + slot, &r, sizeof(r),
                      hipMemcpyHostToDevice, s) != hipSuccess ||
       hipStreamSynchronize(s) != hipSuccess) {
     set_error("fw_batch_set_tissue: copy failed");
     return FW_E_HIP;
   }
+  b->host[slot] = *d;
   return FW_OK;
 }
 
-int fw_batch_set_blocks(fw_batch* b, const int32_t* btis, const int32_t* bstart,
-                        int32_t nblocks, void* stream) {
-  if (!b || nblocks < 0 || (nblocks && (!btis || !bstart))) {
-    set_error("fw_batch_set_blocks: bad arguments");
+int fw_batch_release(fw_batch* b, int32_t slot) {
+  if (!b || slot < 0 || slot >= b->max_tissues) {
+    set_error("fw_batch_release: bad arguments");
     return FW_E_INVALID;
   }
-  for (int32_t q = 0; q < nblocks; ++q) {
-    if (btis[q] < 0 || btis[q] >= b->max_tissues || bstart[q] < 0 ||
-        (int64_t)bstart[q] >= b->host[btis[q]].n) {
-      set_error("fw_batch_set_blocks: block %d points outside its tissue", q);
+  b->host[slot] = fw_tissue_desc();
+  return FW_OK;
+}
+
+int fw_batch_block_nodes(int32_t model, int32_t precision) {
+  if (model < 0 || model >= FW_N_MODELS ||
+      (precision != FW_PREC_FP64 && precision != FW_PREC_FP32)) {
+    set_error("fw_batch_block_nodes: bad arguments");
+    return FW_E_INVALID;
+  }
+  return batch_unroll(model, precision) * kBlock;
+}
+
+int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void* stream) {
+  if (!b || n_slots < 0 || (n_slots && !slots)) {
+    set_error("fw_batch_set_active: bad arguments");
+    return FW_E_INVALID;
+  }
+  const int64_t per = (int64_t)batch_unroll(b->model, b->precision) * kBlock;
+  int64_t nblocks = 0;
+  for (int32_t q = 0; q < n_slots; ++q) {
+    const int32_t s = slots[q];
+    if (s < 0 || s >= b->max_tissues || !b->host[s].u) {
+      set_error("fw_batch_set_active: slot %d is not occupied", s);
       return FW_E_INVALID;
+    }
+    nblocks += (b->host[s].n + per - 1) / per;
+  }
+  if (nblocks > INT32_MAX / 2) {
+    set_error("fw_batch_set_active: too many blocks");
+    return FW_E_INVALID;
+  }
+  int32_t* hb = (int32_t*)malloc(sizeof(int32_t) * 2 * (size_t)(nblocks > 0 ? nblocks : 1));
+  if (!hb) {
+    set_error("fw_batch_set_active: out of host memory");
+    return FW_E_NOMEM;
+  }
+  int32_t* ht = hb;
+  int32_t* hs = hb + nblocks;
+  int64_t q0 = 0;
+  for (int32_t q = 0; q < n_slots; ++q) {
+    const int64_t n = b->host[slots[q]].n;
+    for (int64_t st = 0; st < n; st += per, ++q0) {
+      ht[q0] = slots[q];
+      hs[q0] = (int32_t)st;
     }
   }
   hipStream_t s = as_stream(stream);
   if (nblocks > b->cap_blocks) {
-    if (hipStreamSynchronize(s) != hipSuccess) return FW_E_HIP;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      free(hb);
+      set_error("fw_batch_set_active: stream sync failed");
+      return FW_E_HIP;
+    }
     if (b->btis) (void)hipFree(b->btis);
     if (b->bstart) (void)hipFree(b->bstart);
     b->btis = b->bstart = nullptr;
-    const int32_t cap = nblocks + nblocks / 4 + 64;
+    const int64_t cap = nblocks + nblocks / 4 + 64;
     if (hipMalloc(&b->btis, sizeof(int32_t) * cap) != hipSuccess ||
         hipMalloc(&b->bstart, sizeof(int32_t) * cap) != hipSuccess) {
-      set_error("fw_batch_set_blocks: allocation failed");
+      free(hb);
+      set_error("fw_batch_set_active: allocation failed");
       b->cap_blocks = 0;
+      b->nblocks = 0;
       return FW_E_NOMEM;
     }
-    b->cap_blocks = cap;
+    b->cap_blocks = (int32_t)cap;
     if (b->graph) {
       (void)hipGraphExecDestroy(b->graph);
       b->graph = nullptr;
     }
   }
   if (nblocks &&
-      (hipMemcpyAsync(b->btis, btis, sizeof(int32_t) * nblocks, hipMemcpyHostToDevice, s) !=
+      (hipMemcpyAsync(b->btis, ht, sizeof(int32_t) * nblocks, hipMemcpyHostToDevice, s) !=
            hipSuccess ||
-       hipMemcpyAsync(b->bstart, bstart, sizeof(int32_t) * nblocks, hipMemcpyHostToDevice, s) !=
+       hipMemcpyAsync(b->bstart, hs, sizeof(int32_t) * nblocks, hipMemcpyHostToDevice, s) !=
            hipSuccess ||
        hipStreamSynchronize(s) != hipSuccess)) {
-    set_error("fw_batch_set_blocks: copy failed");
+    free(hb);
+    set_error("fw_batch_set_active: copy failed");
     return FW_E_HIP;
   }
-  b->nblocks = nblocks;
+  free(hb);
+  b->nblocks = (int32_t)nblocks;
   return FW_OK;
 }
 

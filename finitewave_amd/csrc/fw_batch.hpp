@@ -1,15 +1,34 @@
 // fw_batch.hpp -- many 2D tissues stepped by one launch per time step.
 //
 // Each tissue keeps its own HBM buffers (u/u_new pair, state arrays, node
-// codes, activation-time map, probe trace) and a descriptor in a device
-// table.  A block table maps every 256-node block of the launch to one
-// tissue, so tissues of uneven size share a launch without padding, and the
-// host admits or retires tissues by rewriting the block table between
-// launches.  Fields are uniform per tissue (scalar conductivity, one fibre
-// direction), held as four half-step values in the descriptor.  Each tissue
-// counts its own local step (global step - step0), which selects its
-// u/u_new orientation, gates its activation-time recording and indexes its
-// probe trace.
+// codes, activation-time map, probe trace) and a record in a device table.
+// A block table maps every block of the launch to a run of consecutive
+// nodes of one tissue, so tissues of uneven size share a launch without
+// padding, and the host admits or retires tissues by rewriting the block
+// table between launches.  Fields are uniform per tissue (scalar
+// conductivity, one fibre direction), held as four half-step values in the
+// record.  Each tissue counts its own local step (global step - step0),
+// which selects its u/u_new orientation, gates its activation-time
+// recording and indexes its probe trace.
+//
+// The kernel is shaped for a memory-bound step:
+//  * the tissue record is read with uniform (scalar) loads at entry, before
+//    any store; node indices are 32-bit; no node coordinate is divided out;
+//  * for the light models (state all of the arithmetic type, at most three
+//    arrays) a work-item steps 4 consecutive nodes: each row of the 3x6
+//    neighbourhood is one 4-wide vector load plus its two edge values, the
+//    node codes of a row one 32-bit load, the state one vector load per
+//    array, and the results one vector store per array -- a quarter of the
+//    memory instructions of one node per work-item.  Nodes that are not
+//    updated are written back unchanged, so every store covers whole lines;
+//  * the per-node arithmetic is one function (diffuse9 + ionic_update) for
+//    both layouts, the same operations in the same order as the
+//    single-tissue kernel, so results do not depend on the layout.
+// Admission checks that no updated node (code & 3 == 1) lies on the
+// tissue's outer layer and that no node is frozen (code bit 1): the kernel
+// then needs no column guard, and rewriting a non-updated node's u_new with
+// its u is exact (such a node is never stimulated or updated, so both
+// buffers hold its initial value).
 #pragma once
 
 #include "../../include/finitewave_amd.h"
@@ -17,12 +36,24 @@
 
 namespace fw {
 
-// the public descriptor (include/finitewave_amd.h) is the device layout
-typedef fw_tissue_desc TissueDesc;
+// device record of one tissue, built from the public fw_tissue_desc
+struct BatchTissue {
+  void* u;
+  void* un;
+  const uint8_t* code;
+  int32_t* act;
+  void* probe;
+  void* state[FW_MAX_STATE];
+  double dd[4];      // half-step fields (FP64 weights)
+  double act_thr;
+  float df[4];       // the same fields rounded to float (FP32 flux form)
+  float act_thr_f;
+  int32_t n, nj, step0, act_start, probe_idx, probe_len, stencil;
+};
 
 template <typename R>
 struct BatchArgs {
-  const TissueDesc* __restrict__ T;
+  const BatchTissue* __restrict__ T;
   const int32_t* __restrict__ btis;    // tissue of each logical block
   const int32_t* __restrict__ bstart;  // first local node of each logical block
   uint32_t nblocks, per_xcd;
@@ -34,100 +65,271 @@ struct BatchArgs {
   Params P;
 };
 
-// u at a node, 2D, uniform fields: FP64 exact weights or FP32 flux form
+// 3x3 neighbourhood of one node, row-major: NW N NE / W C E / SW S SE
+// (rows are i-1, i, i+1; columns j-1, j, j+1): potentials and conduct bits
 template <typename R>
-__device__ __forceinline__ R batch_diffuse(const TissueDesc& d, const R* __restrict__ u,
-                                           const uint8_t* __restrict__ code, int64_t c,
-                                           R uc, double D, double dtd, double dr2,
-                                           double inv_dr2, R s) {
-  const int64_t sN = d.nj;
-  if (d.stencil == 0) {
-    const int mN = code[c - sN] & 1, mS = code[c + sN] & 1;
-    const int mW = code[c - 1] & 1, mE = code[c + 1] & 1;
+struct Nb9 {
+  R v[9];
+  int m[9];
+};
+
+// diffused u at an interior node of a uniform-field tissue: FP64 exact
+// weights (the reference's weight arithmetic and summation order) or the
+// FP32 flux form
+template <typename R>
+__device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float* df,
+                                      const Nb9<R>& q, double D, double dtd, double dr2,
+                                      double inv_dr2, R s) {
+  const R uc = q.v[4];
+  if (stencil == FW_STENCIL_ISO) {
+    const int mN = q.m[1], mS = q.m[7], mW = q.m[3], mE = q.m[5];
     if constexpr (sizeof(R) == 8) {
       double W[5];
-      W[0] = iso_comp(d.d[0], mN, mS);
-      W[1] = iso_comp(d.d[1], mW, mE);
-      W[3] = iso_comp(d.d[1], mE, mW);
-      W[4] = iso_comp(d.d[0], mS, mN);
+      W[0] = iso_comp(dd[0], mN, mS);
+      W[1] = iso_comp(dd[1], mW, mE);
+      W[3] = iso_comp(dd[1], mE, mW);
+      W[4] = iso_comp(dd[0], mS, mN);
       W[2] = -(W[0] + W[1] + W[3] + W[4]);
       scale_exact<5>(W, 2, D, dtd, dr2, inv_dr2);
-      return u[c - sN] * W[0] + u[c - 1] * W[1] + uc * W[2] + u[c + 1] * W[3] +
-             u[c + sN] * W[4];
+      return q.v[1] * W[0] + q.v[3] * W[1] + uc * W[2] + q.v[5] * W[3] + q.v[7] * W[4];
     } else {
-      const R dx = (R)d.d[0], dy = (R)d.d[1];
+      const R dx = df[0], dy = df[1];
       R fl = R(0);
-      if (mN) fl += dx * (mS ? R(1) : R(2)) * (u[c - sN] - uc);
-      if (mW) fl += dy * (mE ? R(1) : R(2)) * (u[c - 1] - uc);
-      if (mE) fl += dy * (mW ? R(1) : R(2)) * (u[c + 1] - uc);
-      if (mS) fl += dx * (mN ? R(1) : R(2)) * (u[c + sN] - uc);
+      fl += dx * face_w<R>(mN, mS) * (q.v[1] - uc);
+      fl += dy * face_w<R>(mW, mE) * (q.v[3] - uc);
+      fl += dy * face_w<R>(mE, mW) * (q.v[5] - uc);
+      fl += dx * face_w<R>(mS, mN) * (q.v[7] - uc);
       return uc + s * fl;
     }
   }
   Nbr M;
 #pragma unroll
-  for (int q = 0; q < 27; ++q) M.m[q] = 0;
+  for (int k = 0; k < 27; ++k) M.m[k] = 0;
 #pragma unroll
-  for (int di = -1; di <= 1; ++di)
+  for (int di = 0; di < 3; ++di)
 #pragma unroll
-    for (int dj = -1; dj <= 1; ++dj) M.m[(di + 1) * 9 + (dj + 1) * 3 + 1] = code[c + di * sN + dj] & 1;
-  const double dxx = d.d[0], dxy = d.d[1], dyx = d.d[2], dyy = d.d[3];
+    for (int dj = 0; dj < 3; ++dj) M.m[di * 9 + dj * 3 + 1] = q.m[di * 3 + dj];
   if constexpr (sizeof(R) == 8) {
+    const double dxx = dd[0], dxy = dd[1], dyx = dd[2], dyy = dd[3];
     double W[9];
     wts_aniso2d(W, M, dxx, dxx, dxy, dxy, dyx, dyx, dyy, dyy);
     scale_exact<9>(W, 4, D, dtd, dr2, inv_dr2);
-    return u[c - sN - 1] * W[0] + u[c - sN] * W[1] + u[c - sN + 1] * W[2] + u[c - 1] * W[3] +
-           uc * W[4] + u[c + 1] * W[5] + u[c + sN - 1] * W[6] + u[c + sN] * W[7] +
-           u[c + sN + 1] * W[8];
+    return q.v[0] * W[0] + q.v[1] * W[1] + q.v[2] * W[2] + q.v[3] * W[3] + uc * W[4] +
+           q.v[5] * W[5] + q.v[6] * W[6] + q.v[7] * W[7] + q.v[8] * W[8];
   } else {
-    const R uNW = u[c - sN - 1], uN = u[c - sN], uNE = u[c - sN + 1];
-    const R uW = u[c - 1], uE = u[c + 1];
-    const R uSW = u[c + sN - 1], uS = u[c + sN], uSE = u[c + sN + 1];
-    const int mNW = M(-1, -1, 0), mN = M(-1, 0, 0), mNE = M(-1, 1, 0);
-    const int mW = M(0, -1, 0), mE = M(0, 1, 0);
-    const int mSW = M(1, -1, 0), mS = M(1, 0, 0), mSE = M(1, 1, 0);
+    const R dxx = df[0], dxy = df[1], dyx = df[2], dyy = df[3];
+    const R uNW = q.v[0], uN = q.v[1], uNE = q.v[2];
+    const R uW = q.v[3], uE = q.v[5];
+    const R uSW = q.v[6], uS = q.v[7], uSE = q.v[8];
+    const int mNW = q.m[0], mN = q.m[1], mNE = q.m[2];
+    const int mW = q.m[3], mE = q.m[5];
+    const int mSW = q.m[6], mS = q.m[7], mSE = q.m[8];
     R fl = R(0);
-    if (mN) fl += (R)dxx * (uN - uc);
-    fl -= minor_flux<R>((R)dxy, mNW, uNW, mW, uW, mN, uN, 1, uc, mNE, uNE, mE, uE);
-    if (mW) fl += (R)dyy * (uW - uc);
-    fl -= minor_flux<R>((R)dyx, mNW, uNW, mN, uN, mW, uW, 1, uc, mSW, uSW, mS, uS);
-    if (mE) fl += (R)dyy * (uE - uc);
-    fl += minor_flux<R>((R)dyx, mNE, uNE, mN, uN, mE, uE, 1, uc, mSE, uSE, mS, uS);
-    if (mS) fl += (R)dxx * (uS - uc);
-    fl += minor_flux<R>((R)dxy, mSW, uSW, mW, uW, mS, uS, 1, uc, mSE, uSE, mE, uE);
+    fl += dxx * (R)mN * (uN - uc);
+    fl -= minor_flux<R>(dxy, mNW, uNW, mW, uW, mN, uN, 1, uc, mNE, uNE, mE, uE);
+    fl += dyy * (R)mW * (uW - uc);
+    fl -= minor_flux<R>(dyx, mNW, uNW, mN, uN, mW, uW, 1, uc, mSW, uSW, mS, uS);
+    fl += dyy * (R)mE * (uE - uc);
+    fl += minor_flux<R>(dyx, mNE, uNE, mN, uN, mE, uE, 1, uc, mSE, uSE, mS, uS);
+    fl += dxx * (R)mS * (uS - uc);
+    fl += minor_flux<R>(dxy, mSW, uSW, mW, uW, mS, uS, 1, uc, mSE, uSE, mE, uE);
     return uc + s * fl;
   }
 }
 
+// Nodes per work-item: 4 consecutive ones for the light models, else 1; a
+// block covers batch_unroll * kBlock consecutive nodes of one tissue.
+__host__ __device__ constexpr int batch_unroll(int model, int precision) {
+  return model <= 4 ? 4 : 1;
+}
+
+// 4 values of consecutive nodes, element-aligned: one vector load or store
+template <typename T>
+struct Vec4T;
+template <>
+struct Vec4T<float> {
+  typedef float type __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+};
+template <>
+struct Vec4T<double> {
+  typedef double type __attribute__((ext_vector_type(4))) __attribute__((aligned(8)));
+};
+template <>
+struct Vec4T<int32_t> {
+  typedef int32_t type __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+};
+template <typename T>
+using Vec4 = typename Vec4T<T>::type;
+typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
+
 template <int MODEL, typename R>
 __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R> a) {
-  const int64_t lb = logical_block(a.per_xcd);
+  constexpr int NS = model_n_state(MODEL);
+  constexpr int KU = batch_unroll(MODEL, sizeof(R) == 4 ? FW_PREC_FP32 : FW_PREC_FP64);
+  const uint32_t b = blockIdx.x;
+  const uint32_t lb = (b % kXcds) * a.per_xcd + b / kXcds;
   if (lb >= a.nblocks) return;
-  const int t = a.btis[lb];
-  const TissueDesc& d = a.T[t];
-  const int64_t c = (int64_t)a.bstart[lb] + threadIdx.x;
+  // the record, read uniformly before any store
+  const int32_t t = __builtin_amdgcn_readfirstlane(a.btis[lb]);
+  const int32_t start = __builtin_amdgcn_readfirstlane(a.bstart[lb]);
+  const BatchTissue& d = a.T[t];
   const int32_t step = a.step_in ? *a.step_in : a.step;
-  if (a.step_out && lb == 0 && threadIdx.x == 0) *a.step_out = step + 1;
-  if (c >= d.n) return;
   const int32_t ls = step - d.step0;
-  if (ls < 0) return;
-  const R* __restrict__ u = static_cast<const R*>((ls & 1) ? d.un : d.u);
-  R* __restrict__ un = static_cast<R*>((ls & 1) ? d.u : d.un);
-  const uint8_t code = d.code[c];
-  const R uc = u[c];
-  if (d.act && d.act_start >= 0 && ls >= d.act_start) {
-    if (d.act[c] < 0 && uc > (R)d.act_thr) d.act[c] = ls;
+  const int32_t n = d.n, nj = d.nj;
+  const bool act_on = d.act && d.act_start >= 0 && ls >= d.act_start;
+  const bool probe_on = d.probe && ls < d.probe_len;
+  const int32_t probe_idx = d.probe_idx;
+  const int stencil = d.stencil;
+  // pointers read from memory are generic; typed global, their accesses are
+  // global loads and stores, which do not share the scalar loads' counter
+  const global_t<uint8_t>* __restrict__ code = (const global_t<uint8_t>*)d.code;
+  global_t<R>* const ua = (global_t<R>*)d.u;
+  global_t<R>* const ub = (global_t<R>*)d.un;
+  global_t<int32_t>* const act = (global_t<int32_t>*)d.act;
+  global_t<R>* const probe = (global_t<R>*)d.probe;
+  R thr;
+  if constexpr (sizeof(R) == 8) thr = d.act_thr;
+  else thr = d.act_thr_f;
+  double dd[4];
+  float df[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    dd[q] = d.dd[q];
+    df[q] = d.df[q];
   }
-  if (d.probe && c == d.probe_idx && ls < d.probe_len) static_cast<R*>(d.probe)[ls] = uc;
-  if ((code & 3) != 1) return;
-  const int64_t i = c / d.nj, j = c - i * d.nj;
-  if (i <= 0 || i >= d.ni - 1 || j <= 0 || j >= d.nj - 1) return;
   StatePtrs S;
 #pragma unroll
-  for (int k = 0; k < 24; ++k) S.p[k] = d.state[k];
-  R val = batch_diffuse<R>(d, u, d.code, c, uc, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
-  val = ionic_update<MODEL, R>(uc, val, S, c, a.dt, a.P);
-  un[c] = val;
+  for (int k = 0; k < NS; ++k) S.p[k] = d.state[k];
+
+  if (a.step_out && lb == 0 && threadIdx.x == 0) *a.step_out = step + 1;
+  if (ls < 0) return;
+  const global_t<R>* __restrict__ u = (ls & 1) ? ub : ua;
+  global_t<R>* __restrict__ un = (ls & 1) ? ua : ub;
+  const bool aniso = stencil != FW_STENCIL_ISO;
+
+  if constexpr (KU == 1) {
+    const int32_t c = start + (int32_t)threadIdx.x;
+    if (c >= n) return;
+    const uint8_t cd = code[c];
+    const R uc = u[c];
+    if (act_on && act[c] < 0 && uc > thr) act[c] = ls;
+    if (probe_on && c == probe_idx) probe[ls] = uc;
+    // rows 0 and ni-1 would reach outside the grid
+    if ((cd & 3) != 1 || c < nj || c >= n - nj) return;
+    Nb9<R> q;
+#pragma unroll
+    for (int di = -1; di <= 1; ++di)
+#pragma unroll
+      for (int dj = -1; dj <= 1; ++dj) {
+        const int k = (di + 1) * 3 + dj + 1;
+        const bool need = aniso || di == 0 || dj == 0;
+        q.v[k] = need ? u[c + di * nj + dj] : R(0);
+        q.m[k] = need ? (code[c + di * nj + dj] & 1) : 0;
+      }
+    R val = diffuse9<R>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+    val = ionic_update<MODEL, R>(uc, val, S, (int64_t)c, a.dt, a.P);
+    un[c] = val;
+  } else {
+    const int32_t c = start + 4 * (int32_t)threadIdx.x;  // a multiple of 4
+    if (c >= n) return;
+    // the whole 3x6 neighbourhood inside the grid: one vector per row
+    if (c >= nj + 1 && c + 4 + nj < n) {
+      R uv[3][6];
+      int mv[3][6];
+      uint32_t cw_c = 0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const int32_t row = c + (r - 1) * nj;
+        const Vec4<R> v4 = *(const global_t<Vec4<R>>*)(u + row);
+        const uint32_t w = *(const global_t<u32_unaligned>*)(code + row);
+        if (r == 1) cw_c = w;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          uv[r][k + 1] = v4[k];
+          mv[r][k + 1] = (w >> (8 * k)) & 1;
+        }
+        const bool edges = aniso || r == 1;
+        uv[r][0] = edges ? u[row - 1] : R(0);
+        uv[r][5] = edges ? u[row + 4] : R(0);
+        mv[r][0] = edges ? (code[row - 1] & 1) : 0;
+        mv[r][5] = edges ? (code[row + 4] & 1) : 0;
+      }
+      if (act_on) {
+        Vec4<int32_t> a4 = *(const global_t<Vec4<int32_t>>*)(act + c);
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (a4[k] < 0 && uv[1][k + 1] > thr) {
+            a4[k] = ls;
+            hit = true;
+          }
+        if (hit) *(global_t<Vec4<int32_t>>*)(act + c) = a4;
+      }
+      if (probe_on && (uint32_t)(probe_idx - c) < 4u) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          if (c + k == probe_idx) probe[ls] = uv[1][k + 1];
+      }
+      Vec4<R> st[NS > 0 ? NS : 1];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) st[s] = *(const global_t<Vec4<R>>*)((global_t<R>*)S.p[s] + c);
+      Vec4<R> out;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        Nb9<R> q;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int dj = 0; dj < 3; ++dj) {
+            q.v[r * 3 + dj] = uv[r][k + dj];
+            q.m[r * 3 + dj] = mv[r][k + dj];
+          }
+        const R uc = uv[1][k + 1];
+        const bool upd = ((cw_c >> (8 * k)) & 3) == 1;
+        RegState<R, NS> rs;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) rs.v[s] = st[s][k];
+        R val = diffuse9<R>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+        val = ionic_update<MODEL, R>(uc, val, rs, 0, a.dt, a.P);
+        out[k] = upd ? val : uc;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) st[s][k] = upd ? rs.v[s] : st[s][k];
+      }
+      *(global_t<Vec4<R>>*)(un + c) = out;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) *(global_t<Vec4<R>>*)((global_t<R>*)S.p[s] + c) = st[s];
+      return;
+    }
+    // first and last rows and the tissue's tail: node by node
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int32_t cc = c + k;
+      if (cc >= n) break;
+      const uint8_t cd = code[cc];
+      const R uc = u[cc];
+      if (act_on && act[cc] < 0 && uc > thr) act[cc] = ls;
+      if (probe_on && cc == probe_idx) probe[ls] = uc;
+      if ((cd & 3) != 1 || cc < nj || cc >= n - nj) continue;
+      Nb9<R> q;
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) {
+          const int kk = (di + 1) * 3 + dj + 1;
+          const bool need = aniso || di == 0 || dj == 0;
+          q.v[kk] = need ? u[cc + di * nj + dj] : R(0);
+          q.m[kk] = need ? (code[cc + di * nj + dj] & 1) : 0;
+        }
+      RegState<R, NS> rs;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) rs.v[s] = ((const global_t<R>*)S.p[s])[cc];
+      R val = diffuse9<R>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+      val = ionic_update<MODEL, R>(uc, val, rs, 0, a.dt, a.P);
+      un[cc] = val;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) ((global_t<R>*)S.p[s])[cc] = rs.v[s];
+    }
+  }
 }
 
 }  // namespace fw

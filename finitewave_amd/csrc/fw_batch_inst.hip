@@ -22,7 +22,7 @@ using BReal = std::conditional_t<FW_PREC == 0, double, float>;
 
 int FW_BLAUNCHER(const BatchLaunch& L, hipStream_t s) {
   BatchArgs<BReal> a;
-  a.T = static_cast<const TissueDesc*>(L.table);
+  a.T = static_cast<const BatchTissue*>(L.table);
   a.btis = L.btis;
   a.bstart = L.bstart;
   a.nblocks = L.nblocks;

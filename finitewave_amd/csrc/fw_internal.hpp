@@ -89,11 +89,11 @@ struct fw_batch {
   int32_t model, precision, max_tissues;
   double D, dt, dr2, inv_dr2;
   fw::Params P;
-  fw_tissue_desc* table;  // device
+  void* table;            // device fw::BatchTissue[max_tissues]
   int32_t* btis;          // device block table
   int32_t* bstart;
   int32_t nblocks, cap_blocks;
-  int32_t* step_words;
+  int32_t* step_words;    // device int32[4]: graph step words [0..1], border flag [2]
   hipGraphExec_t graph;
   int32_t graph_nblocks;
   fw_tissue_desc* host;   // host mirror of the table

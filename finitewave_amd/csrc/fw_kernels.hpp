@@ -262,13 +262,24 @@ __device__ __forceinline__ R minor_flux(R d, int m0, R u0, int m1, R u1, int m2,
                                         int m5, R u5) {
   const int mh = m2 + m3 + m4 + m5;
   const int ml = m0 + m1 + m2 + m3;
-  if (m2 == 0 || m3 == 0 || mh < 3 || ml < 3) return R(0);
+  const bool on = m2 != 0 && m3 != 0 && mh >= 3 && ml >= 3;
   const R common = u2 + u3;
   const R sh = common + (m4 ? u4 : R(0)) + (m5 ? u5 : R(0));
   const R sl = common + (m0 ? u0 : R(0)) + (m1 ? u1 : R(0));
   const R ih = mh == 4 ? R(0.25) : R(1.0 / 3.0);
   const R il = ml == 4 ? R(0.25) : R(1.0 / 3.0);
-  return d * (sh * ih - sl * il);
+  const R v = d * (sh * ih - sl * il);
+  return on ? v : R(0);  // a select: no branch around the neighbour loads
+}
+
+// FP32 face weight factor, branch-free: 0 when the neighbour does not
+// conduct (the reference's weight is 0 there), 2 when the opposite one does
+// not (its mirror), else 1.  Every neighbour value is loaded unconditionally
+// (interior nodes only, so in bounds) and a face weighted 0 adds exactly 0:
+// no branch divides the wave around the loads.
+template <typename R>
+__device__ __forceinline__ R face_w(int m, int m_opp) {
+  return (R)(m * (2 - m_opp));
 }
 
 // ---------------------------------------------------------------------------
@@ -321,11 +332,12 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
         return u[c - sN] * W[0] + u[c - 1] * W[1] + uc * W[2] + u[c + 1] * W[3] +
                u[c + sN] * W[4];
       } else {
+        const R uN = u[c - sN], uW = u[c - 1], uE = u[c + 1], uS = u[c + sN];
         R fl = R(0);
-        if (mN) fl += (R)dN * (mS ? R(1) : R(2)) * (u[c - sN] - uc);
-        if (mW) fl += (R)dW * (mE ? R(1) : R(2)) * (u[c - 1] - uc);
-        if (mE) fl += (R)dE * (mW ? R(1) : R(2)) * (u[c + 1] - uc);
-        if (mS) fl += (R)dS * (mN ? R(1) : R(2)) * (u[c + sN] - uc);
+        fl += (R)dN * face_w<R>(mN, mS) * (uN - uc);
+        fl += (R)dW * face_w<R>(mW, mE) * (uW - uc);
+        fl += (R)dE * face_w<R>(mE, mW) * (uE - uc);
+        fl += (R)dS * face_w<R>(mS, mN) * (uS - uc);
         return uc + a.s * fl;
       }
     } else {  // anisotropic 9-point
@@ -362,16 +374,16 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
         const int mSW = M(1, -1, 0), mS = M(1, 0, 0), mSE = M(1, 1, 0);
         R fl = R(0);
         // face x0 (i-1/2): major, then the minor with sign -
-        if (mN) fl += (R)dxx0 * (uN - uc);
+        fl += (R)dxx0 * (R)mN * (uN - uc);
         fl -= minor_flux<R>((R)dxy0, mNW, uNW, mW, uW, mN, uN, 1, uc, mNE, uNE, mE, uE);
         // face y0 (j-1/2)
-        if (mW) fl += (R)dyy0 * (uW - uc);
+        fl += (R)dyy0 * (R)mW * (uW - uc);
         fl -= minor_flux<R>((R)dyx0, mNW, uNW, mN, uN, mW, uW, 1, uc, mSW, uSW, mS, uS);
         // face y1 (j+1/2)
-        if (mE) fl += (R)dyy1 * (uE - uc);
+        fl += (R)dyy1 * (R)mE * (uE - uc);
         fl += minor_flux<R>((R)dyx1, mNE, uNE, mN, uN, mE, uE, 1, uc, mSE, uSE, mS, uS);
         // face x1 (i+1/2)
-        if (mS) fl += (R)dxx1 * (uS - uc);
+        fl += (R)dxx1 * (R)mS * (uS - uc);
         fl += minor_flux<R>((R)dxy1, mSW, uSW, mW, uW, mS, uS, 1, uc, mSE, uSE, mE, uE);
         return uc + a.s * fl;
       }
@@ -418,13 +430,15 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
         return u[c - si] * W[0] + u[c - sj] * W[1] + u[c - 1] * W[2] + uc * W[3] +
                u[c + 1] * W[4] + u[c + sj] * W[5] + u[c + si] * W[6];
       } else {
+        const R uXm = u[c - si], uYm = u[c - sj], uZm = u[c - 1];
+        const R uZp = u[c + 1], uYp = u[c + sj], uXp = u[c + si];
         R fl = R(0);
-        if (mXm) fl += (R)dXm * (mXp ? R(1) : R(2)) * (u[c - si] - uc);
-        if (mYm) fl += (R)dYm * (mYp ? R(1) : R(2)) * (u[c - sj] - uc);
-        if (mZm) fl += (R)dZm * (mZp ? R(1) : R(2)) * (u[c - 1] - uc);
-        if (mZp) fl += (R)dZp * (mZm ? R(1) : R(2)) * (u[c + 1] - uc);
-        if (mYp) fl += (R)dYp * (mYm ? R(1) : R(2)) * (u[c + sj] - uc);
-        if (mXp) fl += (R)dXp * (mXm ? R(1) : R(2)) * (u[c + si] - uc);
+        fl += (R)dXm * face_w<R>(mXm, mXp) * (uXm - uc);
+        fl += (R)dYm * face_w<R>(mYm, mYp) * (uYm - uc);
+        fl += (R)dZm * face_w<R>(mZm, mZp) * (uZm - uc);
+        fl += (R)dZp * face_w<R>(mZp, mZm) * (uZp - uc);
+        fl += (R)dYp * face_w<R>(mYp, mYm) * (uYp - uc);
+        fl += (R)dXp * face_w<R>(mXp, mXm) * (uXp - uc);
         return uc + a.s * fl;
       }
     } else {  // anisotropic 19-point
@@ -471,12 +485,24 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
                u[c - si - 1] * W[15] + u[c + si - 1] * W[16] +
                u[c - si + 1] * W[17] + u[c + si + 1] * W[18];
       } else {
+        // the 18 neighbours, loaded unconditionally; U() masks them
+        R un18[27];
+#pragma unroll
+        for (int di = -1; di <= 1; ++di)
+#pragma unroll
+          for (int dj = -1; dj <= 1; ++dj)
+#pragma unroll
+            for (int dk = -1; dk <= 1; ++dk) {
+              const int q = (di + 1) * 9 + (dj + 1) * 3 + dk + 1;
+              const int corner = (di != 0) && (dj != 0) && (dk != 0);
+              un18[q] = (corner || q == 13) ? R(0) : u[c + di * si + dj * sj + dk];
+            }
         auto U = [&](int di, int dj, int dk) -> R {
-          return M(di, dj, dk) ? u[c + di * si + dj * sj + dk] : R(0);
+          return M(di, dj, dk) ? un18[(di + 1) * 9 + (dj + 1) * 3 + dk + 1] : R(0);
         };
         R fl = R(0);
         // x0
-        if (M(-1, 0, 0)) fl += (R)d[0][0] * (U(-1, 0, 0) - uc);
+        fl += (R)d[0][0] * (R)M(-1, 0, 0) * (U(-1, 0, 0) - uc);
         fl -= minor_flux<R>((R)d[0][1], M(-1, -1, 0), U(-1, -1, 0), M(0, -1, 0), U(0, -1, 0),
                             M(-1, 0, 0), U(-1, 0, 0), 1, uc, M(-1, 1, 0), U(-1, 1, 0),
                             M(0, 1, 0), U(0, 1, 0));
@@ -484,7 +510,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
                             M(-1, 0, 0), U(-1, 0, 0), 1, uc, M(-1, 0, 1), U(-1, 0, 1),
                             M(0, 0, 1), U(0, 0, 1));
         // x1
-        if (M(1, 0, 0)) fl += (R)d[1][0] * (U(1, 0, 0) - uc);
+        fl += (R)d[1][0] * (R)M(1, 0, 0) * (U(1, 0, 0) - uc);
         fl += minor_flux<R>((R)d[1][1], M(1, -1, 0), U(1, -1, 0), M(0, -1, 0), U(0, -1, 0),
                             M(1, 0, 0), U(1, 0, 0), 1, uc, M(1, 1, 0), U(1, 1, 0),
                             M(0, 1, 0), U(0, 1, 0));
@@ -492,7 +518,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
                             M(1, 0, 0), U(1, 0, 0), 1, uc, M(1, 0, 1), U(1, 0, 1),
                             M(0, 0, 1), U(0, 0, 1));
         // y0
-        if (M(0, -1, 0)) fl += (R)d[2][0] * (U(0, -1, 0) - uc);
+        fl += (R)d[2][0] * (R)M(0, -1, 0) * (U(0, -1, 0) - uc);
         fl -= minor_flux<R>((R)d[2][1], M(-1, -1, 0), U(-1, -1, 0), M(-1, 0, 0), U(-1, 0, 0),
                             M(0, -1, 0), U(0, -1, 0), 1, uc, M(1, -1, 0), U(1, -1, 0),
                             M(1, 0, 0), U(1, 0, 0));
@@ -500,7 +526,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
                             M(0, -1, 0), U(0, -1, 0), 1, uc, M(0, -1, 1), U(0, -1, 1),
                             M(0, 0, 1), U(0, 0, 1));
         // y1
-        if (M(0, 1, 0)) fl += (R)d[3][0] * (U(0, 1, 0) - uc);
+        fl += (R)d[3][0] * (R)M(0, 1, 0) * (U(0, 1, 0) - uc);
         fl += minor_flux<R>((R)d[3][1], M(-1, 1, 0), U(-1, 1, 0), M(-1, 0, 0), U(-1, 0, 0),
                             M(0, 1, 0), U(0, 1, 0), 1, uc, M(1, 1, 0), U(1, 1, 0),
                             M(1, 0, 0), U(1, 0, 0));
@@ -508,7 +534,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
                             M(0, 1, 0), U(0, 1, 0), 1, uc, M(0, 1, 1), U(0, 1, 1),
                             M(0, 0, 1), U(0, 0, 1));
         // z0
-        if (M(0, 0, -1)) fl += (R)d[4][0] * (U(0, 0, -1) - uc);
+        fl += (R)d[4][0] * (R)M(0, 0, -1) * (U(0, 0, -1) - uc);
         fl -= minor_flux<R>((R)d[4][1], M(-1, 0, -1), U(-1, 0, -1), M(-1, 0, 0), U(-1, 0, 0),
                             M(0, 0, -1), U(0, 0, -1), 1, uc, M(1, 0, -1), U(1, 0, -1),
                             M(1, 0, 0), U(1, 0, 0));
@@ -516,7 +542,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
                             M(0, 0, -1), U(0, 0, -1), 1, uc, M(0, 1, -1), U(0, 1, -1),
                             M(0, 1, 0), U(0, 1, 0));
         // z1
-        if (M(0, 0, 1)) fl += (R)d[5][0] * (U(0, 0, 1) - uc);
+        fl += (R)d[5][0] * (R)M(0, 0, 1) * (U(0, 0, 1) - uc);
         fl += minor_flux<R>((R)d[5][1], M(-1, 0, 1), U(-1, 0, 1), M(-1, 0, 0), U(-1, 0, 0),
                             M(0, 0, 1), U(0, 0, 1), 1, uc, M(1, 0, 1), U(1, 0, 1),
                             M(1, 0, 0), U(1, 0, 0));

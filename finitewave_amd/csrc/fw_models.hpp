@@ -52,18 +52,36 @@ __device__ __forceinline__ float exprel(float y) {
   return fabsf(y) < 1e-4f ? 1.0f + 0.5f * y : expm1f(y) / y;
 }
 
+// state arrays live in global memory: accesses through the global address
+// space compile to global (not flat) loads and stores
 template <typename T>
-__device__ __forceinline__ T& ref(const StatePtrs& S, int k, int64_t c) {
-  return static_cast<T*>(S.p[k])[c];
+using global_t = __attribute__((address_space(1))) T;
+
+template <typename T>
+__device__ __forceinline__ global_t<T>& ref(const StatePtrs& S, int k, int64_t c) {
+  return ((global_t<T>*)S.p[k])[c];
+}
+
+// The state of one node held in registers (models whose state is all of the
+// arithmetic type): the same update code then reads and writes registers,
+// so a kernel can load the state of several nodes before storing any.
+template <typename R, int NS>
+struct RegState {
+  R v[NS > 0 ? NS : 1];
+};
+
+template <typename T, typename R, int NS>
+__device__ __forceinline__ T& ref(RegState<R, NS>& S, int k, int64_t) {
+  return S.v[k];
 }
 
 // ---- Aliev-Panfilov  aliev_panfilov_2d.py:133-203 ---------------------------
-template <typename R>
-__device__ __forceinline__ R ionic_ap(R u, R un, const StatePtrs& S, int64_t c,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_ap(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
   const R a = (R)P.v[0], k = (R)P.v[1], eap = (R)P.v[2], mu_1 = (R)P.v[3],
           mu_2 = (R)P.v[4];
-  R& vref = ref<R>(S, 0, c);
+  auto& vref = ref<R>(S, 0, c);
   R v = vref;
   // calc_v: v += (-dt*(eap + (mu_1*v)/(mu_2 + u)) * (v + k*u*(u - a - 1.)))
   v += (-dt * (eap + (mu_1 * v) / (mu_2 + u)) * (v + k * u * (u - a - R(1))));
@@ -73,11 +91,11 @@ __device__ __forceinline__ R ionic_ap(R u, R un, const StatePtrs& S, int64_t c,
 }
 
 // ---- Barkley  barkley_2d.py:113-166 ----------------------------------------
-template <typename R>
-__device__ __forceinline__ R ionic_barkley(R u, R un, const StatePtrs& S,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_barkley(R u, R un, SP& S,
                                            int64_t c, R dt, const Params& P) {
   const R a = (R)P.v[0], b = (R)P.v[1], eap = (R)P.v[2];
-  R& vref = ref<R>(S, 0, c);
+  auto& vref = ref<R>(S, 0, c);
   R v = vref;
   v += dt * (u - v);
   vref = v;
@@ -85,12 +103,12 @@ __device__ __forceinline__ R ionic_barkley(R u, R un, const StatePtrs& S,
 }
 
 // ---- Mitchell-Schaeffer  mitchell_schaeffer_2d.py:102-217 ------------------
-template <typename R>
-__device__ __forceinline__ R ionic_ms(R u, R un, const StatePtrs& S, int64_t c,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_ms(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
   const R tau_close = (R)P.v[0], tau_open = (R)P.v[1], tau_in = (R)P.v[2],
           tau_out = (R)P.v[3], u_gate = (R)P.v[4];
-  R& href = ref<R>(S, 0, c);
+  auto& href = ref<R>(S, 0, c);
   R h = href;
   h += (u < u_gate) ? dt * (R(1) - h) / tau_open : dt * (-h) / tau_close;
   href = h;
@@ -101,15 +119,15 @@ __device__ __forceinline__ R ionic_ms(R u, R un, const StatePtrs& S, int64_t c,
 }
 
 // ---- Fenton-Karma  fenton_karma_2d.py:143-328 ------------------------------
-template <typename R>
-__device__ __forceinline__ R ionic_fk(R u, R un, const StatePtrs& S, int64_t c,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_fk(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
   const R tau_d = (R)P.v[0], tau_o = (R)P.v[1], tau_r = (R)P.v[2],
           tau_si = (R)P.v[3], tau_v_m = (R)P.v[4], tau_v_p = (R)P.v[5],
           tau_w_m = (R)P.v[6], tau_w_p = (R)P.v[7], k = (R)P.v[8],
           u_c = (R)P.v[9], uc_si = (R)P.v[10];
-  R& vref = ref<R>(S, 0, c);
-  R& wref = ref<R>(S, 1, c);
+  auto& vref = ref<R>(S, 0, c);
+  auto& wref = ref<R>(S, 1, c);
   R v = vref, w = wref;
   const R H1 = (u_c - u) >= 0 ? R(1) : R(0);
   const R H2 = (u - u_c) >= 0 ? R(1) : R(0);
@@ -125,8 +143,8 @@ __device__ __forceinline__ R ionic_fk(R u, R un, const StatePtrs& S, int64_t c,
 }
 
 // ---- Bueno-Orovio  bueno_orovio_2d.py:183-476 -------------------------------
-template <typename R>
-__device__ __forceinline__ R ionic_bo(R u, R un, const StatePtrs& S, int64_t c,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_bo(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
   const R u_o = (R)P.v[0], u_u = (R)P.v[1], theta_v = (R)P.v[2],
           theta_w = (R)P.v[3], theta_v_m = (R)P.v[4], theta_o = (R)P.v[5],
@@ -138,9 +156,9 @@ __device__ __forceinline__ R ionic_bo(R u, R un, const StatePtrs& S, int64_t c,
           tau_s1 = (R)P.v[21], tau_s2 = (R)P.v[22], k_s = (R)P.v[23],
           u_s = (R)P.v[24], tau_si = (R)P.v[25], tau_w_inf = (R)P.v[26],
           w_inf_ = (R)P.v[27];
-  R& vref = ref<R>(S, 0, c);
-  R& wref = ref<R>(S, 1, c);
-  R& sref = ref<R>(S, 2, c);
+  auto& vref = ref<R>(S, 0, c);
+  auto& wref = ref<R>(S, 1, c);
+  auto& sref = ref<R>(S, 2, c);
   R v = vref, w = wref, s = sref;
   // calc_v(v, u, dt, theta_v, calc_v_inf(u), calc_tau_v_m(u), tau_v_p)
   {
@@ -185,8 +203,8 @@ __device__ __forceinline__ R lr91_gate(R var, R dt, R alpha, R beta) {
   return var;
 }
 
-template <typename R>
-__device__ __forceinline__ R ionic_lr91(R u, R un, const StatePtrs& S,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_lr91(R u, R un, SP& S,
                                         int64_t c, R dt, const Params& P) {
   const R gna = (R)P.v[0], gsi = (R)P.v[1], gk = (R)P.v[2], gk1 = (R)P.v[3],
           gkp = (R)P.v[4], gb = (R)P.v[5], ko = (R)P.v[6], ki = (R)P.v[7],
@@ -194,13 +212,13 @@ __device__ __forceinline__ R ionic_lr91(R u, R un, const StatePtrs& S,
           F = (R)P.v[13], PR_NaK = (R)P.v[14];
   // E_Na = R*T/F*np.log(nao/nai) (computed once per launch in the reference)
   const R E_Na = R_ * T / F * flog(nao / nai);
-  R& mr = ref<R>(S, 0, c);
-  R& hr = ref<R>(S, 1, c);
-  R& jr = ref<R>(S, 2, c);
-  R& dr = ref<R>(S, 3, c);
-  R& fr = ref<R>(S, 4, c);
-  R& xr = ref<R>(S, 5, c);
-  R& cair = ref<R>(S, 6, c);
+  auto& mr = ref<R>(S, 0, c);
+  auto& hr = ref<R>(S, 1, c);
+  auto& jr = ref<R>(S, 2, c);
+  auto& dr = ref<R>(S, 3, c);
+  auto& fr = ref<R>(S, 4, c);
+  auto& xr = ref<R>(S, 5, c);
+  auto& cair = ref<R>(S, 6, c);
   // calc_ina
   R ina;
   {
@@ -297,8 +315,8 @@ __device__ __forceinline__ R ionic_lr91(R u, R un, const StatePtrs& S,
 // ---- TP06  tp06_2d.py:242-1103 -----------------------------------------------
 // state order (tp06_2d.py:1073 call): cai casr cass nai Ki m h j xr1 xr2 xs r
 // s d f f2 fcass rr oo ; the first five are double in both precisions.
-template <typename R>
-__device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_tp06(R u, R un, SP& S,
                                         int64_t c, R dt, const Params& P) {
   typedef double D;
   const R ko = (R)P.v[0], cao = (R)P.v[1], nao = (R)P.v[2];
@@ -319,11 +337,11 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
   const D inverseVcF = 1.0 / (Vc * Fd);
   const D inversevssF2 = 1.0 / (2 * Vss * Fd);
 
-  D& cair = ref<D>(S, 0, c);
-  D& casrr = ref<D>(S, 1, c);
-  D& cassr = ref<D>(S, 2, c);
-  D& nair = ref<D>(S, 3, c);
-  D& Kir = ref<D>(S, 4, c);
+  auto& cair = ref<D>(S, 0, c);
+  auto& casrr = ref<D>(S, 1, c);
+  auto& cassr = ref<D>(S, 2, c);
+  auto& nair = ref<D>(S, 3, c);
+  auto& Kir = ref<D>(S, 4, c);
   const D caiD = cair, casrD = casrr, cassD = cassr, naiD = nair, KiD = Kir;
   const R cai = (R)caiD, cass = (R)cassD, nai = (R)naiD, Ki = (R)KiD;
 
@@ -335,9 +353,9 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
   // calc_ina
   R ina;
   {
-    R& mr = ref<R>(S, 5, c);
-    R& hr = ref<R>(S, 6, c);
-    R& jr = ref<R>(S, 7, c);
+    auto& mr = ref<R>(S, 5, c);
+    auto& hr = ref<R>(S, 6, c);
+    auto& jr = ref<R>(S, 7, c);
     const R alpha_m = R(1.0) / (R(1.0) + fexp((R(-60.0) - u) / R(5.0)));
     const R beta_m = R(0.1) / (R(1.0) + fexp((u + R(35.0)) / R(5.0))) +
                      R(0.1) / (R(1.0) + fexp((u - R(50.0)) / R(200.0)));
@@ -380,10 +398,10 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
   // calc_ical
   R ical;
   {
-    R& dr = ref<R>(S, 13, c);
-    R& fr = ref<R>(S, 14, c);
-    R& f2r = ref<R>(S, 15, c);
-    R& fcr = ref<R>(S, 16, c);
+    auto& dr = ref<R>(S, 13, c);
+    auto& fr = ref<R>(S, 14, c);
+    auto& f2r = ref<R>(S, 15, c);
+    auto& fcr = ref<R>(S, 16, c);
     const R d_inf = R(1.0) / (R(1.0) + fexp((R(-8) - u) / R(7.5)));
     const R Ad = R(1.4) / (R(1.0) + fexp((R(-35) - u) / R(13))) + R(0.25);
     const R Bd = R(1.4) / (R(1.0) + fexp((u + R(5)) / R(5)));
@@ -423,8 +441,8 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
   // calc_ito
   R ito;
   {
-    R& rr_ = ref<R>(S, 11, c);
-    R& sr = ref<R>(S, 12, c);
+    auto& rr_ = ref<R>(S, 11, c);
+    auto& sr = ref<R>(S, 12, c);
     const R r_inf = R(1.0) / (R(1.0) + fexp((R(20) - u) / R(6.0)));
     const R s_inf = R(1.0) / (R(1.0) + fexp((u + R(20)) / R(5.0)));
     const R tau_r = R(9.5) * fexp(-(u + R(40.0)) * (u + R(40.0)) / R(1800.0)) + R(0.8);
@@ -439,8 +457,8 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
   // calc_ikr
   R ikr;
   {
-    R& x1r = ref<R>(S, 8, c);
-    R& x2r = ref<R>(S, 9, c);
+    auto& x1r = ref<R>(S, 8, c);
+    auto& x2r = ref<R>(S, 9, c);
     const R xr1_inf = R(1.0) / (R(1.0) + fexp((R(-26.0) - u) / R(7.0)));
     const R axr1 = R(450.0) / (R(1.0) + fexp((R(-45.0) - u) / R(10.0)));
     const R bxr1 = R(6.0) / (R(1.0) + fexp((u - R(-30.0)) / R(11.5)));
@@ -458,7 +476,7 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
   // calc_iks
   R iks;
   {
-    R& xsr = ref<R>(S, 10, c);
+    auto& xsr = ref<R>(S, 10, c);
     const R Axs = R(1400.0) / fsqrt(R(1.0) + fexp((R(5.0) - u) / R(6)));
     const R Bxs = R(1.0) / (R(1.0) + fexp((u - R(35.0)) / R(15.0)));
     const R tau_xs = Axs * Bxs + R(80);
@@ -493,8 +511,8 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, const StatePtrs& S,
   // calc_irel (double: feeds the SR/SS calcium balance)
   D irel;
   {
-    R& rrr = ref<R>(S, 17, c);
-    R& oor = ref<R>(S, 18, c);
+    auto& rrr = ref<R>(S, 17, c);
+    auto& oor = ref<R>(S, 18, c);
     const D kCaSR = maxsr - (maxsr - minsr) / (1 + EC / casrD * (EC / casrD));
     const D k1 = k1_ / kCaSR;
     const D k2 = k2_ * kCaSR;
@@ -550,8 +568,8 @@ __device__ __forceinline__ R crn_gate(R x, R x_inf, R tau_x, R dt) {
   return x_inf - (x_inf - x) * fexp(-dt / tau_x);
 }
 
-template <typename R>
-__device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
+template <typename R, typename SP>
+__device__ __forceinline__ R ionic_crn(R u, R un, SP& S,
                                        int64_t c, R dt, const Params& P) {
   const R gna = (R)P.v[0], gnab = (R)P.v[1], gk1 = (R)P.v[2], gks = (R)P.v[4],
           gto = (R)P.v[5], gcal = (R)P.v[6], gcab = (R)P.v[7],
@@ -565,11 +583,11 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
           trpnmax = (R)P.v[30], cmdnmax = (R)P.v[31], csqnmax = (R)P.v[32],
           inacamax = (R)P.v[33], inakmax = (R)P.v[34], ipcamax = (R)P.v[35],
           krel = (R)P.v[36], iupmax = (R)P.v[37], kq10 = (R)P.v[38];
-  R& nair = ref<R>(S, 0, c);
-  R& kir = ref<R>(S, 1, c);
-  R& cair = ref<R>(S, 2, c);
-  R& caupr = ref<R>(S, 3, c);
-  R& carelr = ref<R>(S, 4, c);
+  auto& nair = ref<R>(S, 0, c);
+  auto& kir = ref<R>(S, 1, c);
+  auto& cair = ref<R>(S, 2, c);
+  auto& caupr = ref<R>(S, 3, c);
+  auto& carelr = ref<R>(S, 4, c);
   const R nai = nair, ki = kir, cai = cair, caup = caupr, carel = carelr;
   // calc_equilibrum_potentials
   const R ena = R_ * T / F * flog(nao / nai);
@@ -578,7 +596,7 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   // calc_gating_m / h / j
   R m, h, j;
   {
-    R& mr = ref<R>(S, 5, c);
+    auto& mr = ref<R>(S, 5, c);
     R am;
     if constexpr (sizeof(R) == 4) {
       am = R(3.2) / exprel(R(-0.1) * (u + R(47.13)));
@@ -596,7 +614,7 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
     mr = m;
   }
   {
-    R& hr = ref<R>(S, 6, c);
+    auto& hr = ref<R>(S, 6, c);
     R ah, bh;
     if (u >= R(-40)) {
       ah = R(0);
@@ -611,7 +629,7 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
     hr = h;
   }
   {
-    R& jr = ref<R>(S, 7, c);
+    auto& jr = ref<R>(S, 7, c);
     R aj, bj;
     if (u >= R(-40)) {
       aj = R(0);
@@ -631,8 +649,8 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   // calc_ito
   R ito;
   {
-    R& oar = ref<R>(S, 10, c);
-    R& oir = ref<R>(S, 11, c);
+    auto& oar = ref<R>(S, 10, c);
+    auto& oir = ref<R>(S, 11, c);
     const R ao = R(0.65) / (fexp(-(u + R(10)) / R(8.5)) + fexp(-(u - R(30)) / R(59.0)));
     const R bo = R(0.65) / (R(2.5) + fexp((u + R(82)) / R(17.0)));
     const R tau_o = R(1) / (kq10 * (ao + bo));
@@ -650,8 +668,8 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   // calc_ikur
   R ikur;
   {
-    R& uar = ref<R>(S, 12, c);
-    R& uir = ref<R>(S, 13, c);
+    auto& uar = ref<R>(S, 12, c);
+    auto& uir = ref<R>(S, 13, c);
     const R gkur = R(0.005) + R(0.05) / (R(1) + fexp(-(u - R(15)) / R(13.0)));
     const R aua = R(0.65) / (fexp(-(u + R(10)) / R(8.5)) + fexp(-(u - R(30)) / R(59.0)));
     const R bua = R(0.65) / (R(2.5) + fexp((u + R(82)) / R(17.0)));
@@ -670,7 +688,7 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   // calc_ikr on the kernel's xr slot (index 15), gkr hard-coded to 0.0294
   R ikr;
   {
-    R& xrr = ref<R>(S, 15, c);
+    auto& xrr = ref<R>(S, 15, c);
     const R gkr = R(0.0294);
     R axr, bxr;
     if constexpr (sizeof(R) == 4) {
@@ -689,7 +707,7 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   // calc_iks on the kernel's xs slot (index 14)
   R iks;
   {
-    R& xsr = ref<R>(S, 14, c);
+    auto& xsr = ref<R>(S, 14, c);
     R axs, bxs;
     if constexpr (sizeof(R) == 4) {
       axs = R(4e-05 * 17) / exprel(-(u - R(19.9)) / R(17));
@@ -707,9 +725,9 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   // calc_ical
   R ical;
   {
-    R& dr = ref<R>(S, 8, c);
-    R& fr = ref<R>(S, 9, c);
-    R& fcar = ref<R>(S, 16, c);
+    auto& dr = ref<R>(S, 8, c);
+    auto& fr = ref<R>(S, 9, c);
+    auto& fcar = ref<R>(S, 16, c);
     R tau_d;
     if constexpr (sizeof(R) == 4) {
       const R y = -(u + R(10)) / R(6.24);
@@ -755,10 +773,10 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
   // calc_irel (uses the irel stored by the previous step)
   R irel;
   {
-    R& irr = ref<R>(S, 17, c);
-    R& vrr = ref<R>(S, 18, c);
-    R& urr = ref<R>(S, 19, c);
-    R& wrr = ref<R>(S, 20, c);
+    auto& irr = ref<R>(S, 17, c);
+    auto& vrr = ref<R>(S, 18, c);
+    auto& urr = ref<R>(S, 19, c);
+    auto& wrr = ref<R>(S, 20, c);
     const R tau_u = R(8);
     const R Fn = R(1e-12) * Vrel * irr - R(5 * 1e-13) / F * (R(0.5) * ical - R(0.2) * inaca);
     const R u_inf = R(1) / (R(1) + fexp(-(Fn - R(3.4175e-13)) / R(1.367e-15)));
@@ -801,9 +819,9 @@ __device__ __forceinline__ R ionic_crn(R u, R un, const StatePtrs& S,
                     inaca + ibna + ibca);
 }
 
-template <int MODEL, typename R>
-__device__ __forceinline__ R ionic_update(R u, R un, const StatePtrs& S,
-                                          int64_t c, R dt, const Params& P) {
+template <int MODEL, typename R, typename SP>
+__device__ __forceinline__ R ionic_update(R u, R un, SP& S, int64_t c, R dt,
+                                          const Params& P) {
   if constexpr (MODEL == 0) return ionic_ap<R>(u, un, S, c, dt, P);
   else if constexpr (MODEL == 1) return ionic_barkley<R>(u, un, S, c, dt, P);
   else if constexpr (MODEL == 2) return ionic_ms<R>(u, un, S, c, dt, P);

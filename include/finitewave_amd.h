@@ -5,7 +5,7 @@
  * pointer (HIP), C-order over the grid shape, unless stated otherwise.  No
  * entry point allocates device memory except fw_plan_create (a small
  * parameter block), fw_batch_create (the tissue descriptor table) and
- * fw_batch_set_blocks (the block table, when it grows).  `stream` is a
+ * fw_batch_set_active (the block table, when it grows).  `stream` is a
  * hipStream_t passed as void* (NULL = the default stream).  All launches are
  * asynchronous on that stream.
  *
@@ -212,8 +212,8 @@ int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr,
  * No counterpart in the reference, which runs one tissue per process; this
  * is the ensemble / serving path built on the same fused step.  Each tissue
  * owns its device buffers and is described by one fw_tissue_desc; a block
- * table maps the launch's 256-node blocks to tissues; fw_batch_create and
- * fw_batch_set_blocks allocate device memory.  Local step of a
+ * table maps the launch's blocks to tissues; fw_batch_create and
+ * fw_batch_set_active allocate device memory.  Local step of a
  * tissue = global step - step0; its u is in `u` at even local steps and in
  * `un` at odd ones.  act records local steps, probe[ls] = u at probe_idx
  * before local step ls, in the batch's precision (double or float). */
@@ -245,11 +245,17 @@ int fw_batch_destroy(fw_batch* batch);
 /* Writes slot's descriptor (host struct copied to the device table). */
 int fw_batch_set_tissue(fw_batch* batch, int32_t slot, const fw_tissue_desc* desc,
                         void* stream);
-/* Replaces the block table (host arrays of nblocks entries: tissue slot and
- * first local node of each 256-node block); grows its device buffer when
- * needed. */
-int fw_batch_set_blocks(fw_batch* batch, const int32_t* block_tissue,
-                        const int32_t* block_start, int32_t nblocks, void* stream);
+/* Sets the tissues that fw_batch_run steps: `slots` (host, n_slots entries)
+ * lists occupied slots.  The library builds the block table from their sizes
+ * (a block covers fw_batch_block_nodes() consecutive nodes of one tissue)
+ * and grows its device buffer when needed. */
+int fw_batch_set_active(fw_batch* batch, const int32_t* slots, int32_t n_slots,
+                        void* stream);
+/* Empties a slot: it can no longer be listed in fw_batch_set_active or
+ * stimulated until fw_batch_set_tissue fills it again. */
+int fw_batch_release(fw_batch* batch, int32_t slot);
+/* Nodes per block of the batched step for (model, precision). */
+int fw_batch_block_nodes(int32_t model, int32_t precision);
 /* n_steps global steps from step0; use_graph replays a captured two-step
  * graph (recaptured when the block count changes). */
 int fw_batch_run(fw_batch* batch, int64_t step0, int32_t n_steps, int32_t use_graph,

@@ -184,3 +184,36 @@ def test_restitution_ensemble_sample(gpu):
     assert apds[-1] > apds[0]
     ref = _single(cls, "fp32", sims[0])
     _check_same(sims[0].result, ref)
+
+
+@pytest.mark.gpu
+def test_batch_abi_rejects_updated_nodes_on_the_outer_layer(gpu):
+    """The batched kernel has no column guard; fw_batch_set_tissue checks the
+    codes of the outer layer on the device and refuses the tissue."""
+    import ctypes
+    import torch
+    from finitewave_amd import _native
+    from finitewave_amd.batch import _TissueDesc
+    b = TissueBatch(fw.AlievPanfilov2D, precision="fp32", max_tissues=2)
+    shape = (8, 9)
+    code = np.zeros(shape, np.uint8)
+    code[1:-1, 1:-1] = 1
+    bufs = [torch.zeros(shape, dtype=torch.float32, device="cuda") for _ in range(3)]
+    for bad_node, ok in (((3, 0), False), ((0, 4), False), ((7, 8), False), ((3, 1), True)):
+        c = code.copy()
+        c[bad_node] = 1
+        dc = torch.from_numpy(c).cuda()
+        desc = _TissueDesc()
+        desc.u, desc.un = bufs[0].data_ptr(), bufs[1].data_ptr()
+        desc.state[0] = bufs[2].data_ptr()
+        desc.code = dc.data_ptr()
+        desc.ni, desc.nj, desc.n = shape[0], shape[1], shape[0] * shape[1]
+        desc.stencil = _native.STENCIL_ISO
+        desc.probe_idx = -1
+        desc.act_start = -1
+        rc = _native.lib().fw_batch_set_tissue(b.handle, 0, ctypes.byref(desc), None)
+        torch.cuda.synchronize()
+        assert (rc == 0) == ok, (bad_node, rc)
+        if not ok:
+            assert b"outer" in _native.lib().fw_last_error()
+    b.close()
