@@ -25,6 +25,7 @@ import argparse
 import json
 import math
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -45,6 +46,18 @@ DR = 0.25  # 0.025 cm in the model's millimetres
 # code per node and the 4-byte activation-time word per node
 B_STATE = {"fp64": 304, "fp32": 188}
 B_TRACK = 4
+
+
+def profiled_traffic(key: str):
+    """HBM bytes per launch of a kernel as measured with rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters (scaled by a device copy of known size)
+    and recorded in profiles/*.md as `traffic_bytes_per_launch <key>: <B>`;
+    (None, None) when no summary holds it."""
+    for p in sorted((ROOT / "profiles").glob("*.md")):
+        m = re.search(rf"^traffic_bytes_per_launch {key}: (\d+)\s*$", p.read_text(), re.M)
+        if m:
+            return int(m.group(1)), f"profiles/{p.name}"
+    return None, None
 
 
 def build_model(precision: str):
@@ -192,6 +205,8 @@ def ensemble_leg(n_tissues: int, steps: int, full: bool):
                roofline={"bound": "hbm", "achieved": round(bytes_step / sec / 1e9, 1),
                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(bytes_step / sec / 1e9 / HBM_PEAK_GBS, 4),
+                         "traffic": profiled_traffic("ensemble")[0] if n_tissues == 512 else None,
+                         "traffic_source": profiled_traffic("ensemble")[1],
                          "bytes_per_step": bytes_step},
                projected_s=round(total_cell_steps / (n_active / sec), 1))
     if full:
@@ -233,7 +248,7 @@ def _fp64_plain(cls, sim):
     return np.array(act.output), np.array(ap.output)
 
 
-def serving_leg(n_req: int, fp64_baseline: bool):
+def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False):
     """The Poisson request stream served with continuous batching (FP32,
     TP06 calcium in FP64), then, for comparison and for the error figures,
     the same requests one after another on the plain FP64 path."""
@@ -262,6 +277,7 @@ def serving_leg(n_req: int, fp64_baseline: bool):
         torch.cuda.synchronize()
         t_plain = 0.0
         lat_max, lat_bad, lat_n, apd_err = 0.0, 0, 0, []
+        per_req = []
         for r in reqs:
             w0 = time.perf_counter()
             act64, trace64 = _fp64_plain(classes[r.model], r.sim)
@@ -278,6 +294,15 @@ def serving_leg(n_req: int, fp64_baseline: bool):
             a64 = apd90(trace64, 0.01, r.sim.apd_threshold)
             if a is not None and a64 is not None:
                 apd_err.append(abs(a - a64))
+            tag = r.sim.tag
+            per_req.append(dict(model=r.model, side=tag["side"], p=round(tag["p"], 3),
+                                cross=tag["cross"], lat_max_ms=float(d.max()) if d.size else 0.0,
+                                lat_p99_ms=float(np.percentile(d, 99)) if d.size else 0.0,
+                                over_1dt=int(np.count_nonzero(d > 0.01 + 1e-9)),
+                                only_one=int(np.count_nonzero((a32 >= 0) != (act64 >= 0))),
+                                nodes=int(both.sum()),
+                                apd90_err_ms=(abs(a - a64) if a is not None and a64 is not None
+                                              else None)))
         plain_rate = n_req / t_plain
         out["fp64_plain"] = dict(sims_per_s=round(plain_rate, 4), wall_s=round(t_plain, 2))
         out["speedup_vs_fp64_plain"] = round(stats["sims_per_s"] / plain_rate, 2)
@@ -285,6 +310,8 @@ def serving_leg(n_req: int, fp64_baseline: bool):
                                      lat_over_1dt=lat_bad,
                                      apd90_max_ms=max(apd_err) if apd_err else None,
                                      apd90_compared=len(apd_err))
+        if per_request:
+            out["per_request"] = per_req
     return out
 
 
@@ -317,6 +344,8 @@ def main(argv=None) -> int:
                     help="also run the whole ensemble to completion")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--no-fp64-serving", action="store_true")
+    ap.add_argument("--per-request", action="store_true",
+                    help="list each served request's errors against the FP64 path")
     ap.add_argument("--legs", default="stencil,ensemble,serving")
     args = ap.parse_args(argv)
     legs = set(args.legs.split(","))
@@ -349,7 +378,9 @@ def main(argv=None) -> int:
             ms_per_step=sec_per_step * 1e3,
             roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                      "traffic": None, "bytes_per_step": bytes_step},
+                      "traffic": profiled_traffic(prec)[0],
+                      "traffic_source": profiled_traffic(prec)[1],
+                      "bytes_per_step": bytes_step},
             finite=r["finite"], activated=r["activated"])
     p = out[primary]
     line = {
@@ -382,7 +413,8 @@ def main(argv=None) -> int:
                                         args.ensemble_full)
         print(f"# ensemble: {line['ensemble']}", file=sys.stderr, flush=True)
     if "serving" in legs:
-        line["serving"] = serving_leg(args.serve_requests, not args.no_fp64_serving)
+        line["serving"] = serving_leg(args.serve_requests, not args.no_fp64_serving,
+                                      args.per_request)
         print(f"# serving: {line['serving']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

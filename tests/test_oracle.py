@@ -243,3 +243,69 @@ def test_model_tables_match_the_product():
         inits = dict(spec["state"])
         for attr, init_attr, _ in cls._state:
             assert getattr(m, init_attr) == inits[attr], (name, attr)
+
+
+def _minor_flux(d, m0, u0, m1, u1, m2, u2, m3, u3, m4, u4, m5, u5):
+    # the FP32 path's minor flux (finitewave_amd/csrc/fw_kernels.hpp minor_flux)
+    mh = m2 + m3 + m4 + m5
+    ml = m0 + m1 + m2 + m3
+    if not (m2 and m3 and mh >= 3 and ml >= 3):
+        return 0.0
+    sh = u2 + u3 + m4 * u4 + m5 * u5
+    sl = u2 + u3 + m0 * u0 + m1 * u1
+    return d * (sh * (0.25 if mh == 4 else 1 / 3) - sl * (0.25 if ml == 4 else 1 / 3))
+
+
+def _flux_form_2d(u, mesh, d, s):
+    """The FP32 path's 9-point flux form, evaluated in float64: the sum of
+    four major face fluxes and eight minor (cross) fluxes of neighbour means."""
+    dxx, dxy, dyx, dyy = d
+    out = u.copy()
+    ni, nj = u.shape
+    for i in range(1, ni - 1):
+        for j in range(1, nj - 1):
+            if mesh[i, j] != 1:
+                continue
+            M = lambda a, b: int(mesh[i + a, j + b] == 1)  # noqa: E731
+            U = lambda a, b: u[i + a, j + b]  # noqa: E731
+            uc = u[i, j]
+            fl = dxx[i - 1, j] * M(-1, 0) * (U(-1, 0) - uc)
+            fl -= _minor_flux(dxy[i - 1, j], M(-1, -1), U(-1, -1), M(0, -1), U(0, -1), M(-1, 0),
+                              U(-1, 0), 1, uc, M(-1, 1), U(-1, 1), M(0, 1), U(0, 1))
+            fl += dyy[i, j - 1] * M(0, -1) * (U(0, -1) - uc)
+            fl -= _minor_flux(dyx[i, j - 1], M(-1, -1), U(-1, -1), M(-1, 0), U(-1, 0), M(0, -1),
+                              U(0, -1), 1, uc, M(1, -1), U(1, -1), M(1, 0), U(1, 0))
+            fl += dyy[i, j] * M(0, 1) * (U(0, 1) - uc)
+            fl += _minor_flux(dyx[i, j], M(-1, 1), U(-1, 1), M(-1, 0), U(-1, 0), M(0, 1),
+                              U(0, 1), 1, uc, M(1, 1), U(1, 1), M(1, 0), U(1, 0))
+            fl += dxx[i, j] * M(1, 0) * (U(1, 0) - uc)
+            fl += _minor_flux(dxy[i, j], M(1, -1), U(1, -1), M(0, -1), U(0, -1), M(1, 0),
+                              U(1, 0), 1, uc, M(1, 1), U(1, 1), M(0, 1), U(0, 1))
+            out[i, j] = uc + s * fl
+    return out
+
+
+@pytest.mark.parametrize("trial", range(6))
+def test_fp32_flux_form_is_the_reference_weighted_sum(trial):
+    """The FP32 path rewrites the 9-point weighted sum as face fluxes (no
+    weights, no weight arrays).  Algebraically the two must agree on every
+    mesh: evaluated in float64 on random fibrotic meshes, random fibre angles
+    and random u, the flux form equals sum_k w_k u_k with the oracle's
+    weights (asymmetric_stencil_2d.py:204-433) to rounding."""
+    rng = np.random.RandomState(100 + trial)
+    shape = (22, 25)
+    mesh = boxed_mesh(shape, fibrosis=rng.uniform(0, 0.45), seed=trial)
+    ang = rng.uniform(0, np.pi)
+    f = np.zeros((*shape, 2))
+    f[..., 0], f[..., 1] = np.cos(ang), np.sin(ang)
+    sim = OracleSim("aliev_panfilov", mesh, 0.01, 0.25, fibers=f, stencil="aniso")
+    sim.initialize()
+    u = rng.uniform(-1, 1, size=shape)
+    w = sim.weights
+    ref = u.copy()
+    for c in sim.idx:
+        i, j = divmod(int(c), shape[1])
+        nb = u[i - 1:i + 2, j - 1:j + 2].reshape(-1)
+        ref[i, j] = sum(w[i, j, k] * nb[k] for k in range(9))
+    got = _flux_form_2d(u, mesh, sim._keep, 0.01 / 0.25 ** 2)
+    assert np.max(np.abs(got - ref)) <= 1e-14
