@@ -113,7 +113,8 @@ def test_fp32_one_step_from_random_state_matches_fp64_oracle(gpu, seed):
     potentials on a fibrotic mesh with a random fibre angle, against the
     oracle's FP64 step: the FP32 path's stencil algebra, masks and minor
     fluxes agree with the reference weights to float rounding.
-    Tolerance: 4e-6 absolute on u in [-1, 1] (about 30 float ulps)."""
+    Tolerance: 1e-5 absolute (about 40 float ulps of the stepped values, which
+    stay within [-2, 2]); a wrong weight or mask is off by 1e-3 or more."""
     from finitewave_amd.engine import Engine
     from tests.scenarios import make_pair
     rng = np.random.RandomState(seed)
@@ -134,4 +135,4 @@ def test_fp32_one_step_from_random_state_matches_fp64_oracle(gpu, seed):
     sim.one_step()
     sim.u, sim.u_new = sim.u_new, sim.u
     act = model.cardiac_tissue.mesh == 1
-    assert float(np.max(np.abs(u_gpu[act] - sim.u[act]))) <= 4e-6
+    assert float(np.max(np.abs(u_gpu[act] - sim.u[act]))) <= 1e-5
