@@ -42,9 +42,10 @@ DR = 0.25  # 0.025 cm in the model's millimetres
 
 # bytes each step must move per active node: state read + state written
 # (TP06 reads u and 18 state arrays and writes u_new and 18 arrays; FP32 path:
-# 14 gates/u as FP32, cai/casr/cass/nai/Ki as FP64), plus the 1-byte node
-# code per node and the 4-byte activation-time word per node
-B_STATE = {"fp64": 304, "fp32": 188}
+# 12 gates as FP32; u/u_new, the sodium gates h, j and cai/casr/cass/nai/Ki as
+# FP64), plus the 1-byte node code per node and the 4-byte activation-time word
+# per node
+B_STATE = {"fp64": 304, "fp32": 212}
 B_TRACK = 4
 
 

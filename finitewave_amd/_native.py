@@ -25,7 +25,7 @@ MAX_STATE, MAX_PARAMS, MAX_FIELDS = 24, 64, 9
 
 # every symbol include/finitewave_amd.h declares
 EXPORTS = (
-    "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64",
+    "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64", "fw_u_is_f64",
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
     "fw_stim_apply", "fw_act_time_update", "fw_batch_create", "fw_batch_destroy",
@@ -87,6 +87,7 @@ def lib():
     L.fw_last_error.restype = ctypes.c_char_p
     L.fw_model_info.argtypes = [i32, POINTER(i32), POINTER(i32)]
     L.fw_state_is_f64.argtypes = [i32, i32, i32]
+    L.fw_u_is_f64.argtypes = [i32, i32]
     L.fw_compute_weights.argtypes = [i32, i32, vp, vp, POINTER(vp), POINTER(i64), vp]
     L.fw_diffusion.argtypes = [i32, i32, vp, vp, vp, vp, i64, POINTER(i64), vp]
     L.fw_ionic.argtypes = [i32, i32, vp, vp, POINTER(vp), i32, vp, i64, dbl, POINTER(dbl),
@@ -121,6 +122,14 @@ def state_is_f64(model_id: int, precision: int, k: int) -> bool:
     rc = lib().fw_state_is_f64(model_id, precision, k)
     if rc < 0:
         check(rc, "fw_state_is_f64")
+    return bool(rc)
+
+
+def u_is_f64(model_id: int, precision: int) -> bool:
+    """Whether u / u_new are stored as float64 for this model and precision."""
+    rc = lib().fw_u_is_f64(model_id, precision)
+    if rc < 0:
+        check(rc, "fw_u_is_f64")
     return bool(rc)
 
 

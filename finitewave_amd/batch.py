@@ -172,7 +172,7 @@ class TissueBatch:
     def footprint(self, shape, horizon_steps) -> int:
         """Device bytes a tissue of `shape` needs (state, codes, maps, probe)."""
         n = int(np.prod(shape))
-        rb = 8 if self.prec == _native.PREC_FP64 else 4
+        rb = 8 if _native.u_is_f64(self.model_id, self.prec) else 4
         st = sum(8 if f else 4 for f in self.f64)
         return n * (2 * rb + st + 1 + 4) + rb * (horizon_steps + 1)
 
@@ -194,7 +194,7 @@ class TissueBatch:
         stencil = _native.STENCIL_ISO if tissue.fibers is None else _native.STENCIL_ANISO
         d4 = _uniform_fields(sim, stencil)
         p = self.proto
-        rd = torch.float64 if self.prec == _native.PREC_FP64 else torch.float32
+        rd = torch.float64 if _native.u_is_f64(self.model_id, self.prec) else torch.float32
         u = torch.full(shape, float(p.init_u), dtype=rd, device=self.dev)
         un = u.clone() if p._copy_u_new else torch.zeros(shape, dtype=rd, device=self.dev)
         inits = {name: getattr(p, init_name) for name, init_name, _ in p._state}

@@ -85,6 +85,15 @@ int fw_model_info(int32_t model, int32_t* n_state, int32_t* n_params) {
   return FW_OK;
 }
 
+int fw_u_is_f64(int32_t model, int32_t precision) {
+  if (model < 0 || model >= FW_N_MODELS ||
+      (precision != FW_PREC_FP64 && precision != FW_PREC_FP32)) {
+    set_error("fw_u_is_f64: bad arguments");
+    return FW_E_INVALID;
+  }
+  return u_is_f64(model, precision) ? 1 : 0;
+}
+
 int fw_state_is_f64(int32_t model, int32_t precision, int32_t k) {
   if (model < 0 || model >= FW_N_MODELS || k < 0 || k >= model_n_state(model)) return FW_E_INVALID;
   return state_is_f64(model, precision, k) ? 1 : 0;
@@ -404,7 +413,8 @@ int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim, void* stream
     set_error("fw_stim_apply: bad arguments");
     return FW_E_INVALID;
   }
-  return launch_stim(plan->desc.precision, u, plan->desc.code, stim, plan->desc.shape,
+  return launch_stim(u_is_f64(plan->desc.model, plan->desc.precision), u, plan->desc.code, stim,
+                     plan->desc.shape,
                      as_stream(stream));
 }
 
@@ -414,7 +424,8 @@ int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr, i
     set_error("fw_act_time_update: bad arguments");
     return FW_E_INVALID;
   }
-  return launch_act(plan->desc.precision, act, u, plan->N, thr, step, as_stream(stream));
+  return launch_act(u_is_f64(plan->desc.model, plan->desc.precision), act, u, plan->N, thr, step,
+                    as_stream(stream));
 }
 
 }  // extern "C"
@@ -750,7 +761,7 @@ int fw_batch_stim(fw_batch* b, int32_t slot, int64_t step, const fw_stim_desc* s
   const int64_t ls = step - d.step0;
   void* u = (ls & 1) ? d.un : d.u;
   const int64_t shape[3] = {d.ni, d.nj, 1};
-  return launch_stim(b->precision, u, d.code, stim, shape, as_stream(stream));
+  return launch_stim(u_is_f64(b->model, b->precision), u, d.code, stim, shape, as_stream(stream));
 }
 
 }  // extern "C"

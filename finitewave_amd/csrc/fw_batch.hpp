@@ -51,7 +51,8 @@ struct BatchTissue {
   int32_t n, nj, step0, act_start, probe_idx, probe_len, stencil;
 };
 
-template <typename R>
+// R: arithmetic type of the ionic update; U: storage type of u / u_new
+template <typename R, typename U = R>
 struct BatchArgs {
   const BatchTissue* __restrict__ T;
   const int32_t* __restrict__ btis;    // tissue of each logical block
@@ -61,7 +62,8 @@ struct BatchArgs {
   int32_t* step_out;
   int32_t step;
   double D, dtd, dr2, inv_dr2;
-  R dt, s;
+  R dt;
+  U s;  // D * dt / dr**2 in the potential's type
   Params P;
 };
 
@@ -74,16 +76,18 @@ struct Nb9 {
 };
 
 // diffused u at an interior node of a uniform-field tissue: FP64 exact
-// weights (the reference's weight arithmetic and summation order) or the
-// FP32 flux form
-template <typename R>
+// weights (the reference's weight arithmetic and summation order, EXACT) or
+// the flux form evaluated in R, the potential's type (double fields when R is
+// double, their float roundings when R is float -- the single-tissue
+// kernel's operands)
+template <typename R, bool EXACT>
 __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float* df,
                                       const Nb9<R>& q, double D, double dtd, double dr2,
                                       double inv_dr2, R s) {
   const R uc = q.v[4];
   if (stencil == FW_STENCIL_ISO) {
     const int mN = q.m[1], mS = q.m[7], mW = q.m[3], mE = q.m[5];
-    if constexpr (sizeof(R) == 8) {
+    if constexpr (EXACT) {
       double W[5];
       W[0] = iso_comp(dd[0], mN, mS);
       W[1] = iso_comp(dd[1], mW, mE);
@@ -93,7 +97,8 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
       scale_exact<5>(W, 2, D, dtd, dr2, inv_dr2);
       return q.v[1] * W[0] + q.v[3] * W[1] + uc * W[2] + q.v[5] * W[3] + q.v[7] * W[4];
     } else {
-      const R dx = df[0], dy = df[1];
+      const R dx = sizeof(R) == 8 ? (R)dd[0] : (R)df[0];
+      const R dy = sizeof(R) == 8 ? (R)dd[1] : (R)df[1];
       R fl = R(0);
       fl += dx * face_w<R>(mN, mS) * (q.v[1] - uc);
       fl += dy * face_w<R>(mW, mE) * (q.v[3] - uc);
@@ -109,7 +114,7 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
   for (int di = 0; di < 3; ++di)
 #pragma unroll
     for (int dj = 0; dj < 3; ++dj) M.m[di * 9 + dj * 3 + 1] = q.m[di * 3 + dj];
-  if constexpr (sizeof(R) == 8) {
+  if constexpr (EXACT) {
     const double dxx = dd[0], dxy = dd[1], dyx = dd[2], dyy = dd[3];
     double W[9];
     wts_aniso2d(W, M, dxx, dxx, dxy, dxy, dyx, dyx, dyy, dyy);
@@ -117,7 +122,10 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
     return q.v[0] * W[0] + q.v[1] * W[1] + q.v[2] * W[2] + q.v[3] * W[3] + uc * W[4] +
            q.v[5] * W[5] + q.v[6] * W[6] + q.v[7] * W[7] + q.v[8] * W[8];
   } else {
-    const R dxx = df[0], dxy = df[1], dyx = df[2], dyy = df[3];
+    const R dxx = sizeof(R) == 8 ? (R)dd[0] : (R)df[0];
+    const R dxy = sizeof(R) == 8 ? (R)dd[1] : (R)df[1];
+    const R dyx = sizeof(R) == 8 ? (R)dd[2] : (R)df[2];
+    const R dyy = sizeof(R) == 8 ? (R)dd[3] : (R)df[3];
     const R uNW = q.v[0], uN = q.v[1], uNE = q.v[2];
     const R uW = q.v[3], uE = q.v[5];
     const R uSW = q.v[6], uS = q.v[7], uSE = q.v[8];
@@ -163,7 +171,9 @@ using Vec4 = typename Vec4T<T>::type;
 typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
 
 template <int MODEL, typename R>
-__global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R> a) {
+__global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<MODEL, R>> a) {
+  using U = UType<MODEL, R>;  // the potential's storage type
+  constexpr bool EXACT = sizeof(R) == 8;
   constexpr int NS = model_n_state(MODEL);
   constexpr int KU = batch_unroll(MODEL, sizeof(R) == 4 ? FW_PREC_FP32 : FW_PREC_FP64);
   const uint32_t b = blockIdx.x;
@@ -183,12 +193,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R> a) {
   // pointers read from memory are generic; typed global, their accesses are
   // global loads and stores, which do not share the scalar loads' counter
   const global_t<uint8_t>* __restrict__ code = (const global_t<uint8_t>*)d.code;
-  global_t<R>* const ua = (global_t<R>*)d.u;
-  global_t<R>* const ub = (global_t<R>*)d.un;
+  global_t<U>* const ua = (global_t<U>*)d.u;
+  global_t<U>* const ub = (global_t<U>*)d.un;
   global_t<int32_t>* const act = (global_t<int32_t>*)d.act;
-  global_t<R>* const probe = (global_t<R>*)d.probe;
-  R thr;
-  if constexpr (sizeof(R) == 8) thr = d.act_thr;
+  global_t<U>* const probe = (global_t<U>*)d.probe;
+  U thr;
+  if constexpr (sizeof(U) == 8) thr = d.act_thr;
   else thr = d.act_thr_f;
   double dd[4];
   float df[4];
@@ -203,30 +213,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R> a) {
 
   if (a.step_out && lb == 0 && threadIdx.x == 0) *a.step_out = step + 1;
   if (ls < 0) return;
-  const global_t<R>* __restrict__ u = (ls & 1) ? ub : ua;
-  global_t<R>* __restrict__ un = (ls & 1) ? ua : ub;
+  const global_t<U>* __restrict__ u = (ls & 1) ? ub : ua;
+  global_t<U>* __restrict__ un = (ls & 1) ? ua : ub;
   const bool aniso = stencil != FW_STENCIL_ISO;
 
   if constexpr (KU == 1) {
     const int32_t c = start + (int32_t)threadIdx.x;
     if (c >= n) return;
     const uint8_t cd = code[c];
-    const R uc = u[c];
+    const U uc = u[c];
     if (act_on && act[c] < 0 && uc > thr) act[c] = ls;
     if (probe_on && c == probe_idx) probe[ls] = uc;
     // rows 0 and ni-1 would reach outside the grid
     if ((cd & 3) != 1 || c < nj || c >= n - nj) return;
-    Nb9<R> q;
+    Nb9<U> q;
 #pragma unroll
     for (int di = -1; di <= 1; ++di)
 #pragma unroll
       for (int dj = -1; dj <= 1; ++dj) {
         const int k = (di + 1) * 3 + dj + 1;
         const bool need = aniso || di == 0 || dj == 0;
-        q.v[k] = need ? u[c + di * nj + dj] : R(0);
+        q.v[k] = need ? u[c + di * nj + dj] : U(0);
         q.m[k] = need ? (code[c + di * nj + dj] & 1) : 0;
       }
-    R val = diffuse9<R>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+    U val = diffuse9<U, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
     val = ionic_update<MODEL, R>(uc, val, S, (int64_t)c, a.dt, a.P);
     un[c] = val;
   } else {
@@ -289,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R> a) {
         RegState<R, NS> rs;
 #pragma unroll
         for (int s = 0; s < NS; ++s) rs.v[s] = st[s][k];
-        R val = diffuse9<R>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+        R val = diffuse9<R, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
         val = ionic_update<MODEL, R>(uc, val, rs, 0, a.dt, a.P);
         out[k] = upd ? val : uc;
 #pragma unroll
@@ -323,7 +333,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R> a) {
       RegState<R, NS> rs;
 #pragma unroll
       for (int s = 0; s < NS; ++s) rs.v[s] = ((const global_t<R>*)S.p[s])[cc];
-      R val = diffuse9<R>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+      R val = diffuse9<R, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
       val = ionic_update<MODEL, R>(uc, val, rs, 0, a.dt, a.P);
       un[cc] = val;
 #pragma unroll

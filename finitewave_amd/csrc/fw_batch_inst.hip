@@ -19,9 +19,10 @@
 namespace fw {
 
 using BReal = std::conditional_t<FW_PREC == 0, double, float>;
+using UReal = UType<FW_MODEL, BReal>;  // storage type of u / u_new
 
 int FW_BLAUNCHER(const BatchLaunch& L, hipStream_t s) {
-  BatchArgs<BReal> a;
+  BatchArgs<BReal, UReal> a;
   a.T = static_cast<const BatchTissue*>(L.table);
   a.btis = L.btis;
   a.bstart = L.bstart;
@@ -35,7 +36,7 @@ int FW_BLAUNCHER(const BatchLaunch& L, hipStream_t s) {
   a.dr2 = L.dr2;
   a.inv_dr2 = L.inv_dr2;
   a.dt = (BReal)L.dt;
-  a.s = (BReal)(L.D * L.dt / L.dr2);
+  a.s = (UReal)(L.D * L.dt / L.dr2);
   a.P = L.P;
   hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal>), dim3(L.per_xcd * kXcds), dim3(kBlock), 0,
                      s, a);

@@ -57,9 +57,9 @@ int launch_diffusion(int ndim, int K, double* u_new, const double* u,
 int launch_ionic(int model, int precision, void* u_new, const void* u,
                  const StatePtrs& st, const int64_t* idx, int64_t n,
                  double dt, const Params& P, hipStream_t s);
-int launch_stim(int precision, void* u, const uint8_t* code,
+int launch_stim(int u_f64, void* u, const uint8_t* code,
                 const fw_stim_desc* stim, const int64_t* shape, hipStream_t s);
-int launch_act(int precision, int32_t* act, const void* u, int64_t n,
+int launch_act(int u_f64, int32_t* act, const void* u, int64_t n,
                double thr, int32_t step, hipStream_t s);
 
 void set_error(const char* fmt, ...);

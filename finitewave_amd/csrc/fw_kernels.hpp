@@ -41,10 +41,12 @@ __device__ __forceinline__ double fval(const Field& f, int64_t i, int64_t j,
   return f.data[c];
 }
 
-template <typename R>
+// R: arithmetic type of the ionic update; U: storage type of u / u_new
+// (UType<MODEL, R>, double for the millivolt-scale models on the FP32 path)
+template <typename R, typename U = R>
 struct KArgs {
-  const R* __restrict__ u;
-  R* __restrict__ u_new;
+  const U* __restrict__ u;
+  U* __restrict__ u_new;
   StatePtrs st;
   const uint8_t* __restrict__ code;
   int64_t ni, nj, nk, N;
@@ -53,10 +55,11 @@ struct KArgs {
   Field f[9];
   const double* __restrict__ w;  // precomputed weights (stencil 2)
   double D, dtd, dr2, inv_dr2;  // inv_dr2: exact 1/dr2 when dr2 = 2^k, else 0
-  R dt, s;
+  R dt;
+  U s;  // D * dt / dr**2 in the potential's type
   Params P;
   int32_t* act;
-  R act_thr;
+  U act_thr;
   int32_t act_on;
   int32_t step;
   const int32_t* step_in;  // graph replay: step read from / written to device
@@ -293,8 +296,11 @@ __device__ __forceinline__ double state_as_double(const StatePtrs& S, int k,
   return (double)static_cast<const float*>(S.p[k])[c];
 }
 
-template <int NDIM, int ST, typename R>
-__device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
+// R here is the potential's storage type (FP32 path of the millivolt-scale
+// models: double); EXACT selects the reference's FP64 weights, else the flux
+// form evaluated in R
+template <int NDIM, int ST, typename RA, typename R, bool EXACT>
+__device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j,
                                      int64_t k, int64_t c, R uc) {
   const int64_t si = a.nj * a.nk, sj = a.nk;  // 2D: nk = 1, sj = 1, si = nj
   const R* __restrict__ u = a.u;
@@ -321,7 +327,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
       const double dS = fval(a.f[0], i, j, 0, c);
       const double dW = fval(a.f[1], i, j - 1, 0, c - 1);
       const double dE = fval(a.f[1], i, j, 0, c);
-      if constexpr (sizeof(R) == 8) {
+      if constexpr (EXACT) {
         double W[5];
         W[0] = iso_comp(dN, mN, mS);
         W[1] = iso_comp(dW, mW, mE);
@@ -358,7 +364,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
       const double dyx1 = fval(a.f[2], i, j, 0, c);
       const double dyy0 = fval(a.f[3], i, j - 1, 0, c - 1);
       const double dyy1 = fval(a.f[3], i, j, 0, c);
-      if constexpr (sizeof(R) == 8) {
+      if constexpr (EXACT) {
         double W[9];
         wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
         scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
@@ -417,7 +423,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
       const double dYp = fval(a.f[1], i, j, k, c);
       const double dZm = fval(a.f[2], i, j, k - 1, c - 1);
       const double dZp = fval(a.f[2], i, j, k, c);
-      if constexpr (sizeof(R) == 8) {
+      if constexpr (EXACT) {
         double W[7];
         W[0] = iso_comp(dXm, mXm, mXp);
         W[1] = iso_comp(dYm, mYm, mYp);
@@ -473,7 +479,7 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
       d[5][0] = fval(a.f[8], i, j, k, c);
       d[5][1] = fval(a.f[6], i, j, k, c);
       d[5][2] = fval(a.f[7], i, j, k, c);
-      if constexpr (sizeof(R) == 8) {
+      if constexpr (EXACT) {
         double W[19];
         wts_aniso3d(W, M, d);
         scale_exact<19>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
@@ -556,13 +562,14 @@ __device__ __forceinline__ R diffuse(const KArgs<R>& a, int64_t i, int64_t j,
 }
 
 template <int NDIM, int ST, int MODEL, typename R, int PREC>
-__global__ __launch_bounds__(kBlock) void k_step(const KArgs<R> a) {
+__global__ __launch_bounds__(kBlock) void k_step(const KArgs<R, UType<MODEL, R>> a) {
+  using U = UType<MODEL, R>;
   const int64_t lb = logical_block(a.per_xcd);
   if (lb >= a.nblocks) return;
   const int64_t c = lb * kBlock + threadIdx.x;
   if (c >= a.N) return;
   const uint8_t code = a.code[c];
-  const R uc = a.u[c];
+  const U uc = a.u[c];
   const int32_t step = a.step_in ? *a.step_in : a.step;
   if (a.act_on) {
     if (a.act[c] < 0 && uc > a.act_thr) a.act[c] = step;
@@ -578,7 +585,7 @@ __global__ __launch_bounds__(kBlock) void k_step(const KArgs<R> a) {
     bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
     if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
     if (interior) {
-      R un = diffuse<NDIM, ST, R>(a, i, j, k, c, uc);
+      U un = diffuse<NDIM, ST, R, U, PREC == 0>(a, i, j, k, c, uc);
       un = ionic_update<MODEL, R>(uc, un, a.st, c, a.dt, a.P);
       a.u_new[c] = un;
     }
@@ -718,8 +725,8 @@ __global__ __launch_bounds__(kBlock) void k_diffusion(double* __restrict__ u_new
 }
 
 template <int MODEL, typename R>
-__global__ __launch_bounds__(kBlock) void k_ionic(R* __restrict__ u_new,
-                                                  const R* __restrict__ u,
+__global__ __launch_bounds__(kBlock) void k_ionic(UType<MODEL, R>* __restrict__ u_new,
+                                                  const UType<MODEL, R>* __restrict__ u,
                                                   const StatePtrs st,
                                                   const int64_t* __restrict__ idx,
                                                   int64_t n, R dt, const Params P) {

@@ -76,8 +76,9 @@ static void ionic_t(int precision, void* u_new, const void* u, const StatePtrs& 
     hipLaunchKernelGGL((k_ionic<MODEL, double>), g, b, 0, s, static_cast<double*>(u_new),
                        static_cast<const double*>(u), st, idx, n, dt, P);
   else
-    hipLaunchKernelGGL((k_ionic<MODEL, float>), g, b, 0, s, static_cast<float*>(u_new),
-                       static_cast<const float*>(u), st, idx, n, (float)dt, P);
+    hipLaunchKernelGGL((k_ionic<MODEL, float>), g, b, 0, s,
+                       static_cast<UType<MODEL, float>*>(u_new),
+                       static_cast<const UType<MODEL, float>*>(u), st, idx, n, (float)dt, P);
 }
 
 int launch_ionic(int model, int precision, void* u_new, const void* u,
@@ -100,7 +101,7 @@ int launch_ionic(int model, int precision, void* u_new, const void* u,
   return check_launch("ionic");
 }
 
-int launch_stim(int precision, void* u, const uint8_t* code, const fw_stim_desc* st,
+int launch_stim(int u_f64, void* u, const uint8_t* code, const fw_stim_desc* st,
                 const int64_t* shape, hipStream_t s) {
   int64_t lo[3], ext[3];
   for (int a = 0; a < 3; ++a) {
@@ -112,7 +113,7 @@ int launch_stim(int precision, void* u, const uint8_t* code, const fw_stim_desc*
   const int64_t n = ext[0] * ext[1] * ext[2];
   if (n == 0) return FW_OK;
   dim3 g(blocks_for(n)), b(kBlock);
-  if (precision == FW_PREC_FP64)
+  if (u_f64)
     hipLaunchKernelGGL((k_stim<double>), g, b, 0, s, static_cast<double*>(u), code, st->mask,
                        lo[0], lo[1], lo[2], ext[0], ext[1], ext[2], shape[1], shape[2],
                        st->kind, st->value, st->clamp, st->u_max);
@@ -123,11 +124,11 @@ int launch_stim(int precision, void* u, const uint8_t* code, const fw_stim_desc*
   return check_launch("stim");
 }
 
-int launch_act(int precision, int32_t* act, const void* u, int64_t n, double thr,
+int launch_act(int u_f64, int32_t* act, const void* u, int64_t n, double thr,
                int32_t step, hipStream_t s) {
   if (n == 0) return FW_OK;
   dim3 g(blocks_for(n)), b(kBlock);
-  if (precision == FW_PREC_FP64)
+  if (u_f64)
     hipLaunchKernelGGL((k_act<double>), g, b, 0, s, act, static_cast<const double*>(u), n,
                        thr, step);
   else

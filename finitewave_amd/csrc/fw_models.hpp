@@ -14,6 +14,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "fw_math.hpp"
 
 namespace fw {
@@ -73,6 +75,24 @@ struct RegState {
 template <typename T, typename R, int NS>
 __device__ __forceinline__ T& ref(RegState<R, NS>& S, int k, int64_t) {
   return S.v[k];
+}
+
+// Rush-Larsen gate update g <- g_inf - (g_inf - g) * exp(-dt / tau) (e.g.
+// tp06_2d.py:307-309).  FP64 path: that expression.  A gate stored double on
+// the FP32 path is advanced by its increment, g + (g_inf - g) * (1 - e) with
+// 1 - e = -expm1(-dt / tau) in float and the sum in double: a float gate
+// near its steady value stops moving once the increment falls below half an
+// ulp, and 1 - e formed from a float e loses its digits for slow gates; both
+// biases shift the resting sodium availability (TP06 j by 2.7e-4 after
+// 780 ms) and with it the conduction velocity.
+template <typename R, typename G>
+__device__ __forceinline__ G rl_gate(G g, R inf, R tau, R dt) {
+  if constexpr (sizeof(R) == 4 && sizeof(G) == 8) {
+    const R em = -expm1f(-dt / tau);
+    return g + ((G)inf - g) * (G)em;
+  } else {
+    return inf - (inf - g) * fexp(-dt / tau);
+  }
 }
 
 // ---- Aliev-Panfilov  aliev_panfilov_2d.py:133-203 ---------------------------
@@ -203,9 +223,10 @@ __device__ __forceinline__ R lr91_gate(R var, R dt, R alpha, R beta) {
   return var;
 }
 
-template <typename R, typename SP>
-__device__ __forceinline__ R ionic_lr91(R u, R un, SP& S,
-                                        int64_t c, R dt, const Params& P) {
+template <typename R, typename SP, typename U = R>
+__device__ __forceinline__ U ionic_lr91(U u_in, U un, SP& S, int64_t c, R dt,
+                                        const Params& P) {
+  const R u = (R)u_in;  // the potential in the arithmetic type
   const R gna = (R)P.v[0], gsi = (R)P.v[1], gk = (R)P.v[2], gk1 = (R)P.v[3],
           gkp = (R)P.v[4], gb = (R)P.v[5], ko = (R)P.v[6], ki = (R)P.v[7],
           nai = (R)P.v[8], nao = (R)P.v[9], R_ = (R)P.v[11], T = (R)P.v[12],
@@ -315,9 +336,10 @@ __device__ __forceinline__ R ionic_lr91(R u, R un, SP& S,
 // ---- TP06  tp06_2d.py:242-1103 -----------------------------------------------
 // state order (tp06_2d.py:1073 call): cai casr cass nai Ki m h j xr1 xr2 xs r
 // s d f f2 fcass rr oo ; the first five are double in both precisions.
-template <typename R, typename SP>
-__device__ __forceinline__ R ionic_tp06(R u, R un, SP& S,
-                                        int64_t c, R dt, const Params& P) {
+template <typename R, typename SP, typename U = R>
+__device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
+                                        const Params& P) {
+  const R u = (R)u_in;  // the potential in the arithmetic type
   typedef double D;
   const R ko = (R)P.v[0], cao = (R)P.v[1], nao = (R)P.v[2];
   const D Vc = P.v[3], Vsr = P.v[4], Vss = P.v[5];
@@ -353,9 +375,11 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, SP& S,
   // calc_ina
   R ina;
   {
+    // h, j: stored double on the FP32 path as well (state_is_f64)
+    typedef std::conditional_t<sizeof(R) == 4, double, R> G;
     auto& mr = ref<R>(S, 5, c);
-    auto& hr = ref<R>(S, 6, c);
-    auto& jr = ref<R>(S, 7, c);
+    auto& hr = ref<G>(S, 6, c);
+    auto& jr = ref<G>(S, 7, c);
     const R alpha_m = R(1.0) / (R(1.0) + fexp((R(-60.0) - u) / R(5.0)));
     const R beta_m = R(0.1) / (R(1.0) + fexp((u + R(35.0)) / R(5.0))) +
                      R(0.1) / (R(1.0) + fexp((u - R(50.0)) / R(200.0)));
@@ -388,11 +412,12 @@ __device__ __forceinline__ R ionic_tp06(R u, R un, SP& S,
     const R tau_j = R(1.0) / (alpha_j + beta_j);
     const R j_inf = h_inf;
     const R m = m_inf - (m_inf - mr) * fexp(-dt / tau_m);
-    const R h = h_inf - (h_inf - hr) * fexp(-dt / tau_h);
-    const R j = j_inf - (j_inf - jr) * fexp(-dt / tau_j);
+    const G hg = rl_gate<R, G>(hr, h_inf, tau_h, dt);
+    const G jg = rl_gate<R, G>(jr, j_inf, tau_j, dt);
     mr = m;
-    hr = h;
-    jr = j;
+    hr = hg;
+    jr = jg;
+    const R h = (R)hg, j = (R)jg;
     ina = gna * m * m * m * h * j * (u - Ena);
   }
   // calc_ical
@@ -568,9 +593,10 @@ __device__ __forceinline__ R crn_gate(R x, R x_inf, R tau_x, R dt) {
   return x_inf - (x_inf - x) * fexp(-dt / tau_x);
 }
 
-template <typename R, typename SP>
-__device__ __forceinline__ R ionic_crn(R u, R un, SP& S,
-                                       int64_t c, R dt, const Params& P) {
+template <typename R, typename SP, typename U = R>
+__device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
+                                        const Params& P) {
+  const R u = (R)u_in;  // the potential in the arithmetic type
   const R gna = (R)P.v[0], gnab = (R)P.v[1], gk1 = (R)P.v[2], gks = (R)P.v[4],
           gto = (R)P.v[5], gcal = (R)P.v[6], gcab = (R)P.v[7],
           gkur_coeff = (R)P.v[8], F = (R)P.v[9], T = (R)P.v[10],
@@ -819,17 +845,34 @@ __device__ __forceinline__ R ionic_crn(R u, R un, SP& S,
                     inaca + ibna + ibca);
 }
 
-template <int MODEL, typename R, typename SP>
-__device__ __forceinline__ R ionic_update(R u, R un, SP& S, int64_t c, R dt,
+// u and u_new are stored as UType<MODEL, R>: the arithmetic type, except on
+// the FP32 path of the millivolt-scale models (LR91, TP06, Courtemanche),
+// whose potential is kept double.  There the step's increment dt * sum(I) at
+// rest is far below half an ulp of a float near -86 mV (3.8e-6 mV), so a
+// float potential stops following the slow relaxation of the resting state,
+// and the resting potential of tissue ahead of a front drifts from the FP64
+// one (measured 1.6e-3 mV), shifting conduction velocity by ~1e-4 and
+// activation times by several steps over a thousand nodes.  With u double the
+// currents and gates stay float and only the final u_new - dt * sum(I) is
+// double.  The FP64 path is unchanged (U = R = double).
+__host__ __device__ constexpr bool u_is_f64(int model, int precision) {
+  return precision == 0 || model >= 5;
+}
+
+template <int MODEL, typename R>
+using UType = std::conditional_t<(sizeof(R) == 8 || MODEL >= 5), double, R>;
+
+template <int MODEL, typename R, typename SP, typename U>
+__device__ __forceinline__ U ionic_update(U u, U un, SP& S, int64_t c, R dt,
                                           const Params& P) {
   if constexpr (MODEL == 0) return ionic_ap<R>(u, un, S, c, dt, P);
   else if constexpr (MODEL == 1) return ionic_barkley<R>(u, un, S, c, dt, P);
   else if constexpr (MODEL == 2) return ionic_ms<R>(u, un, S, c, dt, P);
   else if constexpr (MODEL == 3) return ionic_fk<R>(u, un, S, c, dt, P);
   else if constexpr (MODEL == 4) return ionic_bo<R>(u, un, S, c, dt, P);
-  else if constexpr (MODEL == 5) return ionic_lr91<R>(u, un, S, c, dt, P);
-  else if constexpr (MODEL == 6) return ionic_tp06<R>(u, un, S, c, dt, P);
-  else return ionic_crn<R>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 5) return ionic_lr91<R, SP, U>(u, un, S, c, dt, P);
+  else if constexpr (MODEL == 6) return ionic_tp06<R, SP, U>(u, un, S, c, dt, P);
+  else return ionic_crn<R, SP, U>(u, un, S, c, dt, P);
 }
 
 // number of state arrays per model (state_vars minus u)
@@ -842,8 +885,10 @@ __host__ __device__ constexpr int model_n_params(int model) {
        : model == 4 ? 28 : model == 5 ? 15 : model == 6 ? 49 : 39;
 }
 // FP64 storage of state k under precision prec (1 = FP32 path)
+// TP06 on the FP32 path keeps cai, casr, cass, nai, Ki (k < 5) and the
+// sodium inactivation gates h, j (k = 6, 7) double: see rl_gate
 __host__ __device__ constexpr bool state_is_f64(int model, int prec, int k) {
-  return prec == 0 ? true : (model == 6 && k < 5);
+  return prec == 0 ? true : (model == 6 && (k < 5 || k == 6 || k == 7));
 }
 
 }  // namespace fw

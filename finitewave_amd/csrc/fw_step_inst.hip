@@ -21,9 +21,10 @@
 namespace fw {
 
 using Real = std::conditional_t<FW_PREC == 0, double, float>;
+using UReal = UType<FW_MODEL, Real>;  // storage type of u / u_new
 
 template <int NDIM, int ST>
-static void launch(const fw_plan* plan, const KArgs<Real>& a, hipStream_t s) {
+static void launch(const fw_plan* plan, const KArgs<Real, UReal>& a, hipStream_t s) {
   hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC>), dim3(plan->grid),
                      dim3(kBlock), 0, s, a);
 }
@@ -31,9 +32,9 @@ static void launch(const fw_plan* plan, const KArgs<Real>& a, hipStream_t s) {
 int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
                 const StatePtrs& st, const TrackArgs& tr, hipStream_t s) {
   const fw_plan_desc& d = plan->desc;
-  KArgs<Real> a;
-  a.u = static_cast<const Real*>(u);
-  a.u_new = static_cast<Real*>(u_new);
+  KArgs<Real, UReal> a;
+  a.u = static_cast<const UReal*>(u);
+  a.u_new = static_cast<UReal*>(u_new);
   a.st = st;
   a.code = d.code;
   a.ni = d.shape[0];
@@ -63,10 +64,10 @@ int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
     a.inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
   }
   a.dt = (Real)d.dt;
-  a.s = (Real)(d.D_model * d.dt / d.dr2);
+  a.s = (UReal)(d.D_model * d.dt / d.dr2);
   for (int q = 0; q < 64; ++q) a.P.v[q] = q < d.n_params ? d.params[q] : 0.0;
   a.act = tr.act;
-  a.act_thr = (Real)tr.act_thr;
+  a.act_thr = (UReal)tr.act_thr;
   a.act_on = tr.act_on && tr.act != nullptr;
   a.step = tr.step;
   a.step_in = tr.step_in;

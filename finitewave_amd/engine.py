@@ -165,7 +165,9 @@ class Engine:
         self.ndim = mesh.ndim
         self.N = int(mesh.size)
         self.precision = model._precision_id()
-        self.rdtype = torch.float64 if self.precision == _native.PREC_FP64 else torch.float32
+        # dtype of u / u_new (FP64 also on the FP32 path of LR91/TP06/CRN)
+        self.rdtype = (torch.float64 if _native.u_is_f64(model._model_id(), self.precision)
+                       else torch.float32)
         self.code_base = tissue.node_codes()
         self.code = torch.from_numpy(self.code_base).to(self.dev)
         if getattr(self, "probe_cells", None):

@@ -78,7 +78,8 @@ extern "C" {
 
 /* precision */
 #define FW_PREC_FP64 0 /* all state FP64; bit-for-bit the reference order */
-#define FW_PREC_FP32 1 /* FP32 state; TP06 cai/casr/cass/nai/Ki kept FP64 */
+#define FW_PREC_FP32 1 /* FP32 state; TP06 cai/casr/cass/nai/Ki and the potential of
+                          LR91/TP06/Courtemanche kept FP64 (fw_u_is_f64) */
 
 /* half-step diffusion field storage */
 #define FW_FIELD_UNIFORM 0 /* one value */
@@ -156,6 +157,11 @@ const char* fw_last_error(void);
  * stored as FP64 under `precision` (1) or FP32 (0). */
 int fw_model_info(int32_t model, int32_t* n_state, int32_t* n_params);
 int fw_state_is_f64(int32_t model, int32_t precision, int32_t k);
+/* Whether u / u_new are FP64 (1) or FP32 (0) under `precision`: FP64 on the
+ * FP64 path, and on the FP32 path for LR91, TP06 and Courtemanche, whose
+ * resting potential a float cannot follow (increments below half an ulp near
+ * -86 mV); FP32 for the other models' FP32 path. */
+int fw_u_is_f64(int32_t model, int32_t precision);
 
 /* Weight builder: w[N*K] float64 raw weights (before the D*dt/dr**2 scale),
  * kind FW_STENCIL_ISO / ANISO / SYM; m int8 [N] with 0/1 values;
