@@ -28,9 +28,13 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert L.fw_abi_version() == 1
     assert _native.model_info(_native.MODEL_IDS["tp06"]) == (19, 49)
     assert _native.model_info(_native.MODEL_IDS["courtemanche"]) == (21, 39)
-    # TP06's cai/casr/cass/nai/Ki stay FP64 on the FP32 path
-    assert [_native.state_is_f64(6, 1, k) for k in range(7)] == [True] * 5 + [False] * 2
+    # TP06's cai/casr/cass/nai/Ki and sodium gates h, j stay FP64 on the FP32 path
+    assert [_native.state_is_f64(6, 1, k) for k in range(9)] == \
+        [True] * 5 + [False, True, True, False]
     assert all(_native.state_is_f64(6, 0, k) for k in range(19))
+    # the potential: FP64 on the FP64 path and for LR91/TP06/CRN on the FP32 path
+    assert [_native.u_is_f64(m, 1) for m in range(8)] == [False] * 5 + [True] * 3
+    assert all(_native.u_is_f64(m, 0) for m in range(8))
 
 
 def test_abi_rejects_bad_arguments_without_a_device():
