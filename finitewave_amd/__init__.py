@@ -27,7 +27,10 @@ from .stimulation import (Stim, StimCurrent, StimVoltage, StimSequence, StimCurr
 from .tracker import (Tracker, TrackerSequence, ActivationTime2DTracker,
                       ActivationTime3DTracker, ActionPotential2DTracker,
                       ActionPotential3DTracker, MultiVariable2DTracker, MultiVariable3DTracker,
-                      Variable2DTracker, Variable3DTracker)
+                      Variable2DTracker, Variable3DTracker, LocalActivationTime2DTracker,
+                      LocalActivationTime3DTracker, Period2DTracker, Period3DTracker,
+                      SpiralWaveCore2DTracker, SpiralWaveCore3DTracker, ECG2DTracker,
+                      ECG3DTracker)
 from .command import Command, CommandSequence
 from .state import StateSaver, StateSaverCollection, StateLoader
 from ._native import NativeUnavailableError

@@ -28,7 +28,8 @@ EXPORTS = (
     "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64", "fw_u_is_f64",
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
-    "fw_stim_apply", "fw_act_time_update", "fw_batch_create", "fw_batch_destroy",
+    "fw_stim_apply", "fw_act_time_update", "fw_lat_update", "fw_spiral_tips", "fw_ecg",
+    "fw_batch_create", "fw_batch_destroy",
     "fw_batch_set_tissue", "fw_batch_set_active", "fw_batch_release", "fw_batch_block_nodes",
     "fw_batch_run",
     "fw_batch_stim",
@@ -99,6 +100,9 @@ def lib():
     L.fw_run_graph.argtypes = [vp, vp, vp, POINTER(vp), i64, i32, POINTER(fw_track_desc), vp]
     L.fw_stim_apply.argtypes = [vp, vp, POINTER(fw_stim_desc), vp]
     L.fw_act_time_update.argtypes = [vp, vp, vp, dbl, i32, vp]
+    L.fw_lat_update.argtypes = [vp, vp, vp, vp, i32, dbl, i32, vp, vp]
+    L.fw_spiral_tips.argtypes = [vp, vp, vp, dbl, vp, i32, vp, vp]
+    L.fw_ecg.argtypes = [vp, vp, vp, i32, dbl, vp, vp]
     for name in EXPORTS:
         if name not in ("fw_last_error",):
             getattr(L, name).restype = ctypes.c_int

@@ -38,7 +38,8 @@ def _hipcc() -> str:
 
 def _units(build_dir: Path):
     units = [(CSRC / "fw_abi.hip", build_dir / "fw_abi.o", FP64, []),
-             (CSRC / "fw_literal.hip", build_dir / "fw_literal.o", FP64, [])]
+             (CSRC / "fw_literal.hip", build_dir / "fw_literal.o", FP64, []),
+             (CSRC / "fw_track.hip", build_dir / "fw_track.o", FP64, [])]
     for prec in (0, 1):
         for model in range(8):
             for stem in ("fw_step_inst", "fw_batch_inst"):

@@ -36,6 +36,24 @@ StepLauncher step_launcher(int precision, int model) {
   return table[precision][model];
 }
 
+#define FW_DECL_ECG(p, m)                                                       \
+  int fw_ecg_launch_p##p##_m##m(const fw_plan*, const void*, const double*, int32_t, double, \
+                                double*, hipStream_t);
+FW_DECL_ECG(0, 0) FW_DECL_ECG(0, 1) FW_DECL_ECG(0, 2) FW_DECL_ECG(0, 3)
+FW_DECL_ECG(0, 4) FW_DECL_ECG(0, 5) FW_DECL_ECG(0, 6) FW_DECL_ECG(0, 7)
+FW_DECL_ECG(1, 0) FW_DECL_ECG(1, 1) FW_DECL_ECG(1, 2) FW_DECL_ECG(1, 3)
+FW_DECL_ECG(1, 4) FW_DECL_ECG(1, 5) FW_DECL_ECG(1, 6) FW_DECL_ECG(1, 7)
+
+EcgLauncher ecg_launcher(int precision, int model) {
+  static const EcgLauncher table[2][8] = {
+      {fw_ecg_launch_p0_m0, fw_ecg_launch_p0_m1, fw_ecg_launch_p0_m2, fw_ecg_launch_p0_m3,
+       fw_ecg_launch_p0_m4, fw_ecg_launch_p0_m5, fw_ecg_launch_p0_m6, fw_ecg_launch_p0_m7},
+      {fw_ecg_launch_p1_m0, fw_ecg_launch_p1_m1, fw_ecg_launch_p1_m2, fw_ecg_launch_p1_m3,
+       fw_ecg_launch_p1_m4, fw_ecg_launch_p1_m5, fw_ecg_launch_p1_m6, fw_ecg_launch_p1_m7}};
+  if (precision < 0 || precision > 1 || model < 0 || model > 7) return nullptr;
+  return table[precision][model];
+}
+
 __global__ void k_set_word(int32_t* p, int32_t v) { *p = v; }
 
 // Admission check of a batched tissue's codes: bit 0 of *bad when an updated
@@ -426,6 +444,42 @@ int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr, i
   }
   return launch_act(u_is_f64(plan->desc.model, plan->desc.precision), act, u, plan->N, thr, step,
                     as_stream(stream));
+}
+
+int fw_lat_update(fw_plan* plan, const void* u, int32_t* state, int32_t* steps,
+                  int32_t capacity, double thr, int32_t step, int32_t* overflow, void* stream) {
+  if (!plan || !u || !state || !steps || capacity <= 0 || !overflow) {
+    set_error("fw_lat_update: bad arguments");
+    return FW_E_INVALID;
+  }
+  return launch_lat(u_is_f64(plan->desc.model, plan->desc.precision), u, state, steps, capacity,
+                    thr, step, plan->N, overflow, as_stream(stream));
+}
+
+int fw_spiral_tips(fw_plan* plan, const void* u_prev, const void* u, double thr, double* tips,
+                   int32_t capacity, int32_t* count, void* stream) {
+  if (!plan || !u_prev || !u || !tips || capacity <= 0 || !count) {
+    set_error("fw_spiral_tips: bad arguments");
+    return FW_E_INVALID;
+  }
+  const int64_t shape[3] = {plan->desc.shape[0], plan->desc.shape[1],
+                            plan->desc.ndim == 3 ? plan->desc.shape[2] : 1};
+  return launch_tips(u_is_f64(plan->desc.model, plan->desc.precision), u_prev, u, shape, thr,
+                     tips, capacity, count, as_stream(stream));
+}
+
+int fw_ecg(fw_plan* plan, const void* u, const double* coords, int32_t n_points, double dr,
+           double* out, void* stream) {
+  if (!plan || !u || !coords || n_points <= 0 || !out || !(dr > 0)) {
+    set_error("fw_ecg: bad arguments");
+    return FW_E_INVALID;
+  }
+  EcgLauncher L = ecg_launcher(plan->desc.precision, plan->desc.model);
+  if (!L) {
+    set_error("fw_ecg: no kernel for this plan");
+    return FW_E_UNSUPPORTED;
+  }
+  return L(plan, u, coords, n_points, dr, out, as_stream(stream));
 }
 
 }  // extern "C"

@@ -47,6 +47,17 @@ typedef int (*StepLauncher)(const fw_plan* plan, const void* u, void* u_new,
 // one launcher per (precision, model), defined in fw_step_inst.hip builds
 StepLauncher step_launcher(int precision, int model);
 
+typedef int (*EcgLauncher)(const fw_plan* plan, const void* u, const double* coords,
+                           int32_t n_points, double dr, double* out, hipStream_t stream);
+EcgLauncher ecg_launcher(int precision, int model);
+
+// device trackers (fw_track.hip)
+int launch_lat(int u_f64, const void* u, int32_t* state, int32_t* steps, int32_t cap,
+               double thr, int32_t step, int64_t n, int32_t* overflow, hipStream_t s);
+int launch_tips(int u_f64, const void* u_prev, const void* u, const int64_t* shape, double thr,
+                double* tips, int32_t cap, int32_t* count, hipStream_t s);
+int launch_zero_f64(double* p, int64_t n, hipStream_t s);
+
 // literal sweeps (fw_literal.hip)
 int launch_weights(int ndim, int kind, double* w, const int8_t* m,
                    const double* const* d_dev, const int64_t* shape,

@@ -601,6 +601,70 @@ __global__ __launch_bounds__(kBlock) void k_step(const KArgs<R, UType<MODEL, R>>
   if (a.step_out && c == 0) *a.step_out = step + 1;
 }
 
+// ECG2DTracker / ECG3DTracker (ecg_2d_tracker.py:61-151,
+// ecg_3d_tracker.py:100-140): per measuring point e, the sum over updated
+// nodes of (u_tr - u) / (d_e * dr), u_tr the diffused pre-step potential (the
+// fused step's diffusion value before the ionic update) and d_e the squared
+// index distance (2D: its third coordinate squared added).  Each block sums
+// its nodes with wavefront shuffles and LDS, then adds its partial sums to
+// out[e] with one FP64 atomic per point.
+constexpr int kEcgPoints = 8;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NDIM, int ST, int MODEL, typename R, int PREC>
+__global__ __launch_bounds__(kBlock) void k_ecg(const KArgs<R, UType<MODEL, R>> a,
+                                                const double* __restrict__ coords,
+                                                int32_t n_points, double dr,
+                                                double* __restrict__ out) {
+  using U = UType<MODEL, R>;
+  __shared__ double part[kBlock / 64][kEcgPoints];
+  const int64_t lb = logical_block(a.per_xcd);
+  if (lb >= a.nblocks) return;
+  const int64_t c = lb * kBlock + threadIdx.x;
+  double acc[kEcgPoints];
+#pragma unroll
+  for (int e = 0; e < kEcgPoints; ++e) acc[e] = 0.0;
+  if (c < a.N && (a.code[c] & 3) == 1) {
+    const int64_t sjk = a.nj * a.nk;
+    const int64_t i = c / sjk, rem = c - i * sjk, j = rem / a.nk, k = rem - j * a.nk;
+    bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
+    if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
+    if (interior) {
+      const U uc = a.u[c];
+      const U val = diffuse<NDIM, ST, R, U, PREC == 0>(a, i, j, k, c, uc);
+      const double du = (double)val - (double)uc;
+#pragma unroll
+      for (int e = 0; e < kEcgPoints; ++e) {
+        if (e >= n_points) break;
+        const double x = coords[3 * e], y = coords[3 * e + 1], z = coords[3 * e + 2];
+        const double zz = NDIM == 3 ? z - (double)k : z;
+        const double d = (x - (double)i) * (x - (double)i) + (y - (double)j) * (y - (double)j) +
+                         zz * zz;
+        if (d > 0) acc[e] = du / (d * dr);
+      }
+    }
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int e = 0; e < kEcgPoints; ++e) {
+    if (e >= n_points) break;
+    const double w = wave_sum(acc[e]);
+    if (lane == 0) part[wave][e] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < n_points) {
+    double sum = 0.0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) sum += part[w][threadIdx.x];
+    atomicAdd(out + threadIdx.x, sum);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Literal sweeps (one numba kernel each)
 // ---------------------------------------------------------------------------

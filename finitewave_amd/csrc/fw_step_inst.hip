@@ -29,10 +29,9 @@ static void launch(const fw_plan* plan, const KArgs<Real, UReal>& a, hipStream_t
                      dim3(kBlock), 0, s, a);
 }
 
-int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
-                const StatePtrs& st, const TrackArgs& tr, hipStream_t s) {
+static void fill_args(const fw_plan* plan, const void* u, void* u_new, const StatePtrs& st,
+                      const TrackArgs& tr, KArgs<Real, UReal>& a) {
   const fw_plan_desc& d = plan->desc;
-  KArgs<Real, UReal> a;
   a.u = static_cast<const UReal*>(u);
   a.u_new = static_cast<UReal*>(u_new);
   a.st = st;
@@ -77,6 +76,13 @@ int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
   a.probe_var = tr.probe_var;
   a.probe_buf = tr.probe_buf;
   a.probe_row = tr.probe_buf ? tr.probe_row : -1;
+}
+
+int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
+                const StatePtrs& st, const TrackArgs& tr, hipStream_t s) {
+  const fw_plan_desc& d = plan->desc;
+  KArgs<Real, UReal> a;
+  fill_args(plan, u, u_new, st, tr, a);
   const int st_kind = d.stencil;
   if (d.ndim == 2) {
     if (st_kind == 0) launch<2, 0>(plan, a, s);
@@ -90,6 +96,46 @@ int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("step launch failed: %s", hipGetErrorString(e));
+    return FW_E_HIP;
+  }
+  return FW_OK;
+}
+
+#define FW_ECG_LAUNCHER FW_CAT4(fw_ecg_launch_p, FW_PREC, _m, FW_MODEL)
+
+template <int NDIM, int ST>
+static void launch_ecg_k(const fw_plan* plan, const KArgs<Real, UReal>& a, const double* coords,
+                         int32_t n, double dr, double* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_ecg<NDIM, ST, FW_MODEL, Real, FW_PREC>), dim3(plan->grid), dim3(kBlock),
+                     0, s, a, coords, n, dr, out);
+}
+
+int FW_ECG_LAUNCHER(const fw_plan* plan, const void* u, const double* coords, int32_t n_points,
+                    double dr, double* out, hipStream_t s) {
+  const fw_plan_desc& d = plan->desc;
+  KArgs<Real, UReal> a;
+  TrackArgs none{};
+  StatePtrs st{};
+  fill_args(plan, u, nullptr, st, none, a);
+  int rc = launch_zero_f64(out, n_points, s);
+  if (rc) return rc;
+  for (int32_t e0 = 0; e0 < n_points; e0 += kEcgPoints) {
+    const int32_t m = n_points - e0 < kEcgPoints ? n_points - e0 : kEcgPoints;
+    const double* cp = coords + 3 * e0;
+    double* op = out + e0;
+    if (d.ndim == 2) {
+      if (d.stencil == 0) launch_ecg_k<2, 0>(plan, a, cp, m, dr, op, s);
+      else if (d.stencil == 1) launch_ecg_k<2, 1>(plan, a, cp, m, dr, op, s);
+      else launch_ecg_k<2, 2>(plan, a, cp, m, dr, op, s);
+    } else {
+      if (d.stencil == 0) launch_ecg_k<3, 0>(plan, a, cp, m, dr, op, s);
+      else if (d.stencil == 1) launch_ecg_k<3, 1>(plan, a, cp, m, dr, op, s);
+      else launch_ecg_k<3, 2>(plan, a, cp, m, dr, op, s);
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("ecg launch failed: %s", hipGetErrorString(e));
     return FW_E_HIP;
   }
   return FW_OK;

@@ -204,14 +204,18 @@ class CardiacModel:
     # -- the step loop -----------------------------------------------------------
     def _loop(self, eng, iters: int) -> None:
         from .engine import Probes
-        from .tracker import ActivationTime2DTracker, _ProbeTracker
+        from .tracker import ActivationTime2DTracker, _DeviceTracker, _ProbeTracker
         stims = list(self.stim_sequence.sequence) if self.stim_sequence else []
         trackers = list(self.tracker_sequence.sequence) if self.tracker_sequence else []
         commands = list(self.command_sequence.sequence) if self.command_sequence else []
         act_trackers = [t for t in trackers if isinstance(t, ActivationTime2DTracker)]
         probe_trackers = [t for t in trackers if isinstance(t, _ProbeTracker)]
+        dev_trackers = [t for t in trackers if isinstance(t, _DeviceTracker)]
         host_trackers = [t for t in trackers
-                         if not isinstance(t, (ActivationTime2DTracker, _ProbeTracker))]
+                         if not isinstance(t, (ActivationTime2DTracker, _ProbeTracker,
+                                               _DeviceTracker))]
+        for tr in dev_trackers:
+            tr._device_begin(self, eng)
         torch = eng.torch
         # activation-time maps live on the device during the run
         for tr in act_trackers:
@@ -267,6 +271,7 @@ class CardiacModel:
             extra_open = [tr for tr in extra_act if _gate_open(tr, t, step)]
             probe_open = [tr for tr in probe_trackers if _gate_open(tr, t, step)]
             host_open = [tr for tr in host_trackers if _gate_open(tr, t, step)]
+            dev_open = [tr for tr in dev_trackers if _gate_open(tr, t, step)]
             t_next = t + dt
             cmd_due = any((not c.passed) and t_next >= c.t for c in commands)
             saver_due = False
@@ -278,7 +283,8 @@ class CardiacModel:
             row = -1
             if probe_open:
                 row = len(chunk_flags) if not (due_stims or extra_open or host_open) else 0
-            plain = not (due_stims or extra_open or host_open or cmd_due or saver_due)
+            plain = not (due_stims or extra_open or host_open or dev_open or cmd_due or
+                         saver_due)
             self._t_hist.append(t)
             if plain:
                 chunk_flags.append(1 if act_open else 0)
@@ -297,6 +303,9 @@ class CardiacModel:
             for s in due_stims:
                 s._apply_device(self, eng)
                 s.update_status(self)
+            # device trackers see the pre-step potential after this step's stimuli
+            for tr in dev_open:
+                tr._device_track(self, eng)
             chunk_flags.append(1 if act_open else 0)
             chunk_rows.append(0 if probe_open else -1)
             chunk_row_owners.append(probe_open)
@@ -375,6 +384,11 @@ class CardiacModel:
                 tr._append(data[r, a:a + n])
 
     def _finish_trackers(self, eng) -> None:
+        from .tracker import _DeviceTracker
         for tr, dev_steps in eng.act.items():
             tr._store_steps(dev_steps.cpu().numpy(), self._t_hist)
         eng.act = {}
+        trackers = list(self.tracker_sequence.sequence) if self.tracker_sequence else []
+        for tr in trackers:
+            if isinstance(tr, _DeviceTracker) and getattr(tr, "_dev", None) is not None:
+                tr._device_end(self, eng)

@@ -37,6 +37,11 @@
  *                          cpuwave3D/stimulation/*.py)
  *   fw_act_time_update  <- ActivationTime2DTracker._track
  *                          (cpuwave2D/tracker/activation_time_2d_tracker.py:61-63)
+ *   fw_lat_update       <- LocalActivationTime2DTracker._track
+ *                          (local_activation_time_2d_tracker.py:60-102)
+ *   fw_spiral_tips      <- _track_tip_line (spiral_wave_core_2d_tracker.py:211-249)
+ *   fw_ecg              <- ECG2DTracker.calc_ecg + compute_ecg
+ *                          (ecg_2d_tracker.py:61-151)
  *   fw_batch_*          <- no counterpart (the reference steps one tissue per
  *                          process); many CardiacModel2D.run() calls of one
  *                          model sharing each step's launch.
@@ -213,6 +218,37 @@ int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim,
 /* One activation-time pass over all N nodes (act = step where u > thr first) */
 int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr,
                        int32_t step, void* stream);
+
+/* ---- device trackers: read the potential as it stands before a step ------ */
+
+/* LocalActivationTime2D/3DTracker._track and cross_threshold
+ * (cpuwave2D/tracker/local_activation_time_2d_tracker.py:60-102): a node
+ * crosses when u >= thr while not activated (its crossing step is appended to
+ * steps[c * capacity + k] and it becomes activated) and is deactivated when
+ * u < thr.  state: int32 [N], bit 0 activated, bits 1.. crossings so far,
+ * zero-initialised by the caller.  *overflow (device) is set to 1 when a
+ * node crosses more than `capacity` times.  The reference's layer structure
+ * follows from the crossing steps and is rebuilt by the caller. */
+int fw_lat_update(fw_plan* plan, const void* u, int32_t* state, int32_t* steps,
+                  int32_t capacity, double thr, int32_t step, int32_t* overflow,
+                  void* stream);
+
+/* SpiralWaveCore2D/3DTracker._track (spiral_wave_core_2d_tracker.py:70-82,
+ * 211-249; spiral_wave_core_3d_tracker.py:18-31): tips between u_prev and u
+ * on every slice k, written as tips[5 * q + {0..4}] = (i, j, k, x, y) with
+ * x = j + dy, y = i + dx as the reference appends them; *count (device) = number
+ * found (entries beyond capacity are dropped; order is not the reference's,
+ * sort by (k, i, j)). */
+int fw_spiral_tips(fw_plan* plan, const void* u_prev, const void* u, double thr,
+                   double* tips, int32_t capacity, int32_t* count, void* stream);
+
+/* ECG2D/3DTracker.calc_ecg (ecg_2d_tracker.py:61-151, ecg_3d_tracker.py:
+ * 100-140): out[e] = sum over updated nodes of (u_tr - u) / (d_e * dr), u_tr
+ * the plan's diffusion of u, d_e the squared index distance to coords[3e..3e+2]
+ * (2D: plus z^2).  coords, out: device.  Summation order differs from the
+ * reference's parallel loop (block sums, FP64 atomics). */
+int fw_ecg(fw_plan* plan, const void* u, const double* coords, int32_t n_points,
+           double dr, double* out, void* stream);
 
 /* ---- batches: many 2D tissues of one model, one launch per step ----------
  * No counterpart in the reference, which runs one tissue per process; this

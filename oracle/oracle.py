@@ -213,7 +213,9 @@ class OStim:
 
 @dataclass
 class OTracker:
-    """kind 'act_time' (threshold) or 'probe' (cells, var)."""
+    """kind 'act_time' (threshold), 'probe' (cells, var), 'lat' (threshold;
+    LocalActivationTime), 'period' (cells, threshold), 'ecg' (coords) or
+    'spiral' (threshold; SpiralWaveCore)."""
     kind: str
     threshold: float = -40.0
     cells: list = field(default_factory=list)
@@ -222,6 +224,7 @@ class OTracker:
     end_time: float = math.inf
     step: int = 1
     data: object = None
+    coords: object = None
 
 
 @dataclass
@@ -289,6 +292,10 @@ class OracleSim:
         for tr in self.trackers:
             if tr.kind == "act_time":
                 tr.data = -np.ones_like(self.u)
+            elif tr.kind in ("lat", "period"):
+                tr.data = None
+            elif tr.kind == "spiral":
+                tr.data = {"u_prev": self.u.copy(), "tips": []}
             else:
                 tr.data = []
 
@@ -372,6 +379,36 @@ class OracleSim:
         fn(_p(self.u_new), _p(self.u), _p(self.weights), _p(self.idx, _I64),
            len(self.idx), *self.mesh.shape)
 
+    def _ecg(self, coords) -> np.ndarray:
+        """ECG2DTracker.calc_ecg / compute_ecg (ecg_2d_tracker.py:61-151,
+        ecg_3d_tracker.py:100-140): one diffusion sweep of the pre-step u
+        into a zero array, then per measuring point the sum over active nodes
+        of (u_tr - u) / (d * dr), d the squared index distance, summed in
+        index order."""
+        L = lib()
+        K = self.weights.shape[-1]
+        fn = {(2, 5): L.orc_diffusion_iso2d, (2, 9): L.orc_diffusion_aniso2d,
+              (3, 7): L.orc_diffusion_iso3d, (3, 19): L.orc_diffusion_aniso3d}[(self.dim, K)]
+        u_tr = np.zeros_like(self.u)
+        fn(_p(u_tr), _p(self.u), _p(self.weights), _p(self.idx, _I64),
+           len(self.idx), *self.mesh.shape)
+        shape = self.mesh.shape
+        co = np.unravel_index(self.idx, shape)
+        diff = (u_tr.reshape(-1)[self.idx] - self.u.reshape(-1)[self.idx])
+        out = np.zeros(len(coords))
+        for e, (x, y, z) in enumerate(np.atleast_2d(coords)):
+            if self.dim == 2:
+                d = (x - co[0]) ** 2 + (y - co[1]) ** 2 + z ** 2
+            else:
+                d = (x - co[0]) ** 2 + (y - co[1]) ** 2 + (z - co[2]) ** 2
+            d = d.astype(np.float64)
+            ok = d > 0
+            acc = 0.0
+            for v in (diff[ok] / (d[ok] * self.dr)):
+                acc += v
+            out[e] = acc
+        return out
+
     def _ionic_args(self):
         names = [n for n, _ in self.spec["state"]]
         key = tuple(self.state[n].ctypes.data for n in names)
@@ -417,6 +454,12 @@ class OracleSim:
                 if tr.kind == "act_time":
                     L.orc_act_time(_p(tr.data), _p(self.u), self.u.size, float(tr.threshold),
                                    float(self.t))
+                elif tr.kind in ("lat", "period"):
+                    _lat_track(tr, self.u, self.t)
+                elif tr.kind == "ecg":
+                    tr.data.append(self._ecg(tr.coords))
+                elif tr.kind == "spiral":
+                    _spiral_track(tr, self.u, self.t, self.step)
                 else:
                     src = self.u if tr.var == "u" else self.state[tr.var]
                     cells = tuple(np.atleast_2d(tr.cells).T)
@@ -432,6 +475,104 @@ class OracleSim:
                 on_step(self)
             if (self.t > t_max) or (self.step > max_iters):
                 break
+
+
+def _lat_track(tr, u, t) -> None:
+    """LocalActivationTime2DTracker._track / cross_threshold
+    (local_activation_time_2d_tracker.py:60-102); kind 'period' restricts the
+    layers to tr.cells (period_2d_tracker.py:45-61).  tr.data holds
+    {'act_t': [layers], 'activated': bool array}."""
+    if tr.data is None:
+        n = len(np.atleast_2d(tr.cells)) if tr.kind == "period" else None
+        tr.data = {"act_t": [-np.ones(n if n is not None else u.shape)],
+                   "activated": np.zeros(u.shape, dtype=bool)}
+    st = tr.data
+    cross = (u >= tr.threshold) & (~st["activated"])
+    st["activated"] = np.where(cross, True, st["activated"])
+    back = (u < tr.threshold) & st["activated"]
+    st["activated"] = np.where(back, False, st["activated"])
+    if tr.kind == "period":
+        cross = cross[tuple(np.atleast_2d(tr.cells).T)]
+    if np.any(st["act_t"][-1][cross] > -1):
+        st["act_t"].append(-np.ones(cross.shape))
+    st["act_t"][-1] = np.where(cross, t, st["act_t"][-1])
+
+
+def _tip_cross(u, i, j, thr) -> int:
+    # _apply_threshold (spiral_wave_core_2d_tracker.py:175-208)
+    if u[i, j] >= thr and (u[i + 1, j] < thr or u[i, j + 1] < thr or u[i + 1, j + 1] < thr):
+        return 1
+    if u[i, j] < thr and (u[i + 1, j] >= thr or u[i, j + 1] >= thr or u[i + 1, j + 1] >= thr):
+        return 1
+    return 0
+
+
+def _tip_pos(i, j, u, un, thr):
+    # _correct_tip_pos (spiral_wave_core_2d_tracker.py:108-172), same operation order
+    AC = u[i, j] - u[i, j + 1] + u[i + 1, j + 1] - u[i + 1, j]
+    GC = u[i, j + 1] - u[i, j]
+    BC = u[i + 1, j] - u[i, j]
+    DC = u[i, j] - thr
+    AD = un[i, j] - un[i, j + 1] + un[i + 1, j + 1] - un[i + 1, j]
+    GD = un[i, j + 1] - un[i, j]
+    BD = un[i + 1, j] - un[i, j]
+    DD = un[i, j] - thr
+    Q = BC * AD - BD * AC
+    R_ = GC * AD - GD * AC
+    S = DC * AD - DD * AC
+    QOnR = Q / R_
+    SOnR = S / R_
+    T = AC * QOnR
+    U = AC * SOnR - BC + GC * QOnR
+    V = GC * SOnR - DC
+    disc = U * U - 4. * T * V
+    if disc < 0:
+        return None
+    T2 = 2. * T
+    if T2 == 0.:
+        return None
+    xn = (-U - math.sqrt(disc)) / T2
+    xp = (-U + math.sqrt(disc)) / T2
+    yn = -QOnR * xn - SOnR
+    yp = -QOnR * xp - SOnR
+    if 0 <= xn <= 1 and 0 <= yn <= 1:
+        return (xn, yn)
+    if 0 <= xp <= 1 and 0 <= yp <= 1:
+        return (xp, yp)
+    return None
+
+
+def _tips_2d(u_prev, u, thr) -> list:
+    # _track_tip_line (spiral_wave_core_2d_tracker.py:211-249): margin 5
+    out = []
+    ni, nj = u.shape
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for i in range(5, ni - 5):
+            for j in range(5, nj - 5):
+                c = _tip_cross(u_prev, i, j, thr)
+                if c == 1:
+                    c += _tip_cross(u, i, j, thr)
+                if c == 2:
+                    pos = _tip_pos(i, j, u_prev, u, thr)
+                    if pos is not None:
+                        out.append((j + pos[1], i + pos[0]))
+    return out
+
+
+def _spiral_track(tr, u, t, step) -> None:
+    """SpiralWaveCore2DTracker._track (spiral_wave_core_2d_tracker.py:70-82)
+    and its 3D per-slice form (spiral_wave_core_3d_tracker.py:18-31); the
+    previous u starts as the initial u (initialize, :36-47).  tr.data holds
+    {'u_prev': array, 'tips': [(x, y, (z,) t, step)]}."""
+    st = tr.data
+    if u.ndim == 2:
+        for x, y in _tips_2d(st["u_prev"], u, tr.threshold):
+            st["tips"].append((x, y, t, step))
+    else:
+        for k in range(u.shape[2]):
+            for x, y in _tips_2d(st["u_prev"][:, :, k], u[:, :, k], tr.threshold):
+                st["tips"].append((x, y, k, t, step))
+    st["u_prev"] = u.copy()
 
 
 def half_step_fields(model_D_al, model_D_ac, fibers, conductivity, shape, dim):

@@ -281,3 +281,69 @@ def test_conduction_velocity_closed_form():
     assert abs(conduction_velocity(act, 1, dr) - dr / 0.5) < 1e-12
     act[1, 30] = -1
     assert conduction_velocity(act, 1, dr) is None
+
+
+# ---------------------------------------------------------------- device trackers (host side)
+
+def test_lat_layer_rebuild_equals_the_reference_rule_step_by_step():
+    """The device keeps each node's crossing calls; the host rebuilds the
+    reference's layers from them.  On random potentials (many crossings,
+    re-entries) the rebuilt layers equal the oracle's step-by-step
+    restatement of LocalActivationTime2DTracker._track."""
+    from finitewave_amd.tracker import _lat_layers
+    from oracle.oracle import OTracker, _lat_track
+    rng = np.random.RandomState(4)
+    shape, steps, thr, cap = (7, 9), 300, 0.5, 300
+    u_seq = (rng.random_sample((steps, *shape)) < 0.3).astype(np.float64)
+    u_seq += 0.1 * rng.random_sample((steps, *shape))
+    times = np.arange(steps) * 0.01
+    tr = OTracker("lat", threshold=thr)
+    # device bookkeeping (k_lat) emulated per node
+    n = int(np.prod(shape))
+    state = np.zeros(n, dtype=np.int64)
+    st_steps = np.zeros((n, cap), dtype=np.int64)
+    for k in range(steps):
+        _lat_track(tr, u_seq[k], times[k])
+        u = u_seq[k].reshape(-1)
+        act = state & 1
+        cross = (u >= thr) & (act == 0)
+        cnt = state >> 1
+        st_steps[np.flatnonzero(cross), cnt[cross]] = k
+        state = np.where(cross, ((cnt + 1) << 1) | 1, state)
+        state = np.where((u < thr) & (act == 1), state & ~1, state)
+    got = _lat_layers(state >> 1, st_steps, times, shape)
+    ref = tr.data["act_t"]
+    assert len(got) == len(ref) and len(ref) > 5
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+def test_period_tracker_host_rule_equals_oracle():
+    """Period2DTracker's host replay over probe samples equals the oracle's
+    period tracker on the same samples."""
+    from oracle.oracle import OTracker, _lat_track
+    rng = np.random.RandomState(8)
+    cells = [[1, 2], [3, 4], [5, 1]]
+    steps = 200
+    u_full = (rng.random_sample((steps, 7, 9)) < 0.25).astype(np.float64)
+    t_hist = list(np.arange(steps) * 0.1)
+
+    class M:
+        pass
+    m = M()
+    m._t_hist = []
+    tr = fw.Period2DTracker()
+    tr.cell_ind = cells
+    tr.threshold = 0.5
+    tr.initialize(m)  # as model.initialize does, before any step
+    m._t_hist = t_hist
+    otr = OTracker("period", threshold=0.5, cells=cells)
+    cidx = tuple(np.atleast_2d(cells).T)
+    for k in range(steps):
+        tr._append(u_full[k][cidx])
+        _lat_track(otr, u_full[k], t_hist[k])
+    for a, b in zip(tr.act_t, otr.data["act_t"]):
+        assert np.array_equal(a, b)
+    assert len(tr.act_t) == len(otr.data["act_t"])
+    periods = tr.output
+    assert len(periods) == len(cells)

@@ -309,3 +309,79 @@ def test_fp32_flux_form_is_the_reference_weighted_sum(trial):
         ref[i, j] = sum(w[i, j, k] * nb[k] for k in range(9))
     got = _flux_form_2d(u, mesh, sim._keep, 0.01 / 0.25 ** 2)
     assert np.max(np.abs(got - ref)) <= 1e-14
+
+
+# ---- the reference's tracker known-answer tests (tests/test_trackers_2d.py,
+# tests/test_trackers_3d.py), run on the oracle's trackers ------------------
+
+def _barkley_spiral(dim=2):
+    # spiral_model fixture (test_trackers_2d.py:25-39, test_trackers_3d.py:28-44)
+    shape = (100, 100) if dim == 2 else (100, 100, 3)
+    mesh = boxed_mesh(shape)
+    sim = OracleSim("barkley", mesh, 0.01, 0.25)
+    box1 = (slice(0, 100), slice(0, 3)) + ((slice(0, 3),) if dim == 3 else ())
+    box2 = (slice(0, 50), slice(0, 100)) + ((slice(0, 3),) if dim == 3 else ())
+    sim.stims.append(OStim("voltage", 0, 1, box1))
+    sim.stims.append(OStim("voltage", 5, 1, box2))
+    return sim
+
+
+def _period_mean(layers, t_cells):
+    lats = np.array(layers).T
+    per = [np.diff(row[row != -1]) for row in lats]
+    return np.mean(np.array([np.mean(x) if len(x) > 0 else np.nan for x in per]))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_period_tracker_window(dim):
+    # test_spiral_wave_period_2d_tracker / _3d_tracker: mean period 3.5 +- 0.2
+    sim = _barkley_spiral(dim)
+    cells = [[80, 80], [20, 70], [40, 10], [25, 90]]
+    if dim == 3:
+        cells = [c + [1] for c in cells]
+    tr = OTracker("period", threshold=0.5, cells=cells, start_time=10, step=10)
+    sim.trackers.append(tr)
+    sim.run(20)
+    assert abs(_period_mean(tr.data["act_t"], cells) - 3.5) <= 0.2
+
+
+def test_oracle_spiral_core_tracker_window():
+    # test_spiral_wave_core_2d_tracker: tips within x 32..38, y 47..53
+    sim = _barkley_spiral(2)
+    tr = OTracker("spiral", threshold=0.5, start_time=12, step=10)
+    sim.trackers.append(tr)
+    sim.run(20)
+    tips = np.array(tr.data["tips"])
+    assert len(tips) > 0
+    x, y = tips[:, 0], tips[:, 1]
+    assert x.min() >= 32 and x.max() <= 38 and y.min() >= 47 and y.max() <= 53
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_oracle_ecg_tracker_window(dim):
+    # planar_model + test_ecg_2d_tracker / test_ecg_3d_tracker
+    shape = (50, 5) if dim == 2 else (50, 5, 3)
+    sim = OracleSim("aliev_panfilov", boxed_mesh(shape), 0.0015, 0.25)
+    box = (slice(0, 5), slice(0, 5)) + ((slice(0, 3),) if dim == 3 else ())
+    sim.stims.append(OStim("current", 5, 5, box, duration=0.5))
+    tr = OTracker("ecg", step=10, coords=np.array([[25, 2, 0 if dim == 2 else 1]]))
+    sim.trackers.append(tr)
+    sim.run(15)
+    ecg = np.array(tr.data).T[0]
+    assert ecg.max() > 0.001 and ecg.min() < -0.001 and np.argmax(ecg) > 100
+
+
+def test_oracle_local_activation_time_two_waves():
+    # test_local_activation_time_2d_tracker: two layers, both waves 1.5..2 node speed
+    sim = OracleSim("aliev_panfilov", boxed_mesh((12, 3)), 0.01, 0.25)
+    sim.stims.append(OStim("current", 0, 5, (slice(0, 5), slice(0, 3)), duration=0.5))
+    sim.stims.append(OStim("voltage", 45, 1, (slice(0, 5), slice(0, 10))))
+    tr = OTracker("lat", threshold=0.5)
+    sim.trackers.append(tr)
+    sim.run(50)
+    lats = np.array(tr.data["act_t"])
+    assert len(lats) == 2
+    lat1, lat2 = lats[:, 10, 1]
+    assert lat1 < lat2
+    assert 1.5 <= 5 * 0.25 / lat1 <= 2
+    assert 1.5 <= 5 * 0.25 / (lat2 - 45) <= 2
