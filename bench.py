@@ -209,7 +209,9 @@ def ensemble_leg(n_tissues: int, steps: int, full: bool):
                          "traffic": profiled_traffic("ensemble")[0] if n_tissues == 512 else None,
                          "traffic_source": profiled_traffic("ensemble")[1],
                          "bytes_per_step": bytes_step},
-               projected_s=round(total_cell_steps / (n_active / sec), 1))
+               projected_s=round(total_cell_steps / (n_active / sec), 1),
+               launch_variant=b.variant,
+               tuning_ms={str(k): round(v, 4) for k, v in (b.tuning or {}).items()})
     if full:
         w0 = time.perf_counter()
         b.run_all()

@@ -31,6 +31,7 @@ EXPORTS = (
     "fw_stim_apply", "fw_act_time_update", "fw_lat_update", "fw_spiral_tips", "fw_ecg",
     "fw_batch_create", "fw_batch_destroy",
     "fw_batch_set_tissue", "fw_batch_set_active", "fw_batch_release", "fw_batch_block_nodes",
+    "fw_batch_variants", "fw_batch_set_variant", "fw_batch_get_variant",
     "fw_batch_run",
     "fw_batch_stim",
 )

@@ -17,8 +17,11 @@ node before every step.
 from __future__ import annotations
 
 import ctypes
+import json
 import math
+import os
 from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 
@@ -114,11 +117,52 @@ def stim_schedule(stims, n_steps: int, fold: np.ndarray) -> list:
     return [(k, st) for k, _, st in ev]
 
 
+_TUNE_CACHE: dict = {}
+
+
+def _tune_cache_path() -> Path:
+    return Path(os.environ.get("FW_AUTOTUNE_CACHE",
+                               Path.home() / ".cache" / "finitewave_amd" / "batch_tune.json"))
+
+
+def _tune_cache_load() -> None:
+    if _TUNE_CACHE:
+        return
+    try:
+        _TUNE_CACHE.update(json.loads(_tune_cache_path().read_text()))
+    except (OSError, ValueError):
+        pass
+
+
+def _tune_cache_store(key: str, variant: int, timings: dict) -> None:
+    _TUNE_CACHE[key] = variant
+    p = _tune_cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        data = {}
+        if p.exists():
+            data = json.loads(p.read_text())
+        data[key] = variant
+        data[key + "|ms"] = {str(v): t for v, t in timings.items()}
+        p.write_text(json.dumps(data, indent=1, sort_keys=True))
+    except (OSError, ValueError):
+        pass  # the cache is an optimisation; a read-only home does not stop the run
+
+
 class TissueBatch:
-    """Many tissues of `model_cls` (a finitewave_amd 2D model class)."""
+    """Many tissues of `model_cls` (a finitewave_amd 2D model class).
+
+    Launch shape: the batched kernel has variants (nodes per work-item,
+    block-to-XCD map; `fw_batch_variants`) that compute the same results bit
+    for bit.  With ``autotune`` the batch times each on its own next steps
+    (``tune_steps`` each after one untimed step), keeps the fastest and
+    remembers it per (device, model, precision, size class) in memory and in
+    a JSON cache (``FW_AUTOTUNE_CACHE``, default
+    ``~/.cache/finitewave_amd/batch_tune.json``)."""
 
     def __init__(self, model_cls, precision: str = "fp32", dt: float = 0.01, dr: float = 0.25,
-                 max_tissues: int = 1024, use_graph: bool = True, chunk_cap: int = 4000):
+                 max_tissues: int = 1024, use_graph: bool = True, chunk_cap: int = 4000,
+                 autotune: bool = True, tune_steps: int = 16):
         self.torch = _native.require_gpu()
         self.dev = self.torch.device("cuda", self.torch.cuda.current_device())
         self.proto = model_cls()
@@ -143,6 +187,10 @@ class TissueBatch:
         L.fw_batch_set_active.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                           ctypes.c_void_p]
         L.fw_batch_release.argtypes = [ctypes.c_void_p, ctypes.c_int32]
+        L.fw_batch_variants.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]
+        L.fw_batch_set_variant.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p]
+        L.fw_batch_get_variant.argtypes = [ctypes.c_void_p]
+        L.fw_batch_block_nodes.argtypes = [ctypes.c_void_p]
         L.fw_batch_run.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32,
                                    ctypes.c_int32, ctypes.c_void_p]
         L.fw_batch_stim.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
@@ -150,6 +198,9 @@ class TissueBatch:
         check(L.fw_batch_create(self.model_id, self.prec, float(self.proto.D_model), dt,
                                 dr ** 2, par, len(params), max_tissues,
                                 ctypes.byref(self.handle)), "fw_batch_create")
+        self.autotune, self.tune_steps = autotune, tune_steps
+        self.tuned = not autotune
+        self.tuning: dict | None = None   # variant -> ms per step, when measured here
         self.step = 0                     # global step of the next launch
         self.sims: dict[int, BatchSim] = {}
         self.buffers: dict[int, dict] = {}
@@ -289,6 +340,64 @@ class TissueBatch:
                     check(lib().fw_batch_stim(self.handle, slot, self.step, ctypes.byref(d),
                                               self._stream()), "fw_batch_stim")
 
+    # ------------------------------------------------------------------ launch shape
+    def variants(self) -> list:
+        L = lib()
+        n = L.fw_batch_variants(self.model_id, None, 0)
+        arr = (ctypes.c_int32 * max(1, n))()
+        L.fw_batch_variants(self.model_id, arr, n)
+        return list(arr[:n])
+
+    @property
+    def variant(self) -> int:
+        return lib().fw_batch_get_variant(self.handle)
+
+    def set_variant(self, variant: int) -> None:
+        check(lib().fw_batch_set_variant(self.handle, int(variant), self._stream()),
+              "fw_batch_set_variant")
+
+    def _tune_key(self) -> str:
+        n = sum(int(self.buffers[s]["desc"].n) for s in self.sims)
+        dev = self.torch.cuda.get_device_name(self.dev)
+        return f"{dev}|m{self.model_id}|p{self.prec}|n2^{int(math.log2(max(1, n)))}"
+
+    def _tune(self, n_avail: int) -> int:
+        """Times every variant on the next steps; returns the steps run."""
+        _tune_cache_load()
+        key = self._tune_key()
+        if key in _TUNE_CACHE:
+            self.set_variant(_TUNE_CACHE[key])
+            self.tuned = True
+            return 0
+        vs = self.variants()
+        per = self.tune_steps + 1
+        if len(vs) < 2:
+            self.tuned = True
+            return 0
+        if n_avail < per * len(vs):
+            return 0
+        torch = self.torch
+        L = lib()
+        timings = {}
+        step = self.step
+        for v in vs:
+            self.set_variant(v)
+            check(L.fw_batch_run(self.handle, step, 1, 0, self._stream()), "fw_batch_run")
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            check(L.fw_batch_run(self.handle, step + 1, self.tune_steps, 0, self._stream()),
+                  "fw_batch_run")
+            e1.record()
+            e1.synchronize()
+            timings[v] = e0.elapsed_time(e1) / self.tune_steps
+            step += per
+        best = min(timings, key=timings.get)
+        self.set_variant(best)
+        self.tuning = timings
+        self.tuned = True
+        _tune_cache_store(key, best, timings)
+        return step - self.step
+
     def advance(self, max_steps: int | None = None) -> list:
         """Run until the next admission point: at most `max_steps` steps (or
         the chunk cap), stopping early at stimulus steps and completions.
@@ -307,9 +416,14 @@ class TissueBatch:
             if ev:
                 target = min(target, sim.step0 + ev[0][0])
         n = max(1, target - self.step)
-        check(lib().fw_batch_run(self.handle, self.step, n, 1 if self.use_graph else 0,
-                                 self._stream()), "fw_batch_run")
-        self.step += n
+        if not self.tuned:
+            done = self._tune(n)
+            self.step += done
+            n -= done
+        if n > 0:
+            check(lib().fw_batch_run(self.handle, self.step, n, 1 if self.use_graph else 0,
+                                     self._stream()), "fw_batch_run")
+            self.step += n
         for slot, sim in list(self.sims.items()):
             if self.step - sim.step0 >= sim.n_steps:
                 finished.append(self.retire(slot))

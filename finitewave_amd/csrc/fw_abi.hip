@@ -516,6 +516,7 @@ static BatchLaunch batch_args(const fw_batch* b, int32_t step, const int32_t* in
   L.step_in = in;
   L.step_out = out;
   L.step = step;
+  L.variant = b->variant;
   L.D = b->D;
   L.dt = b->dt;
   L.dr2 = b->dr2;
@@ -544,6 +545,8 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   b->model = model;
   b->precision = precision;
   b->max_tissues = max_tissues;
+  b->variant = batch_default_variant(model);
+  b->active = new (std::nothrow) int32_t[max_tissues];
   b->D = D_model;
   b->dt = dt;
   b->dr2 = dr2;
@@ -552,7 +555,8 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   b->inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
   for (int q = 0; q < FW_MAX_PARAMS; ++q) b->P.v[q] = q < n_params ? params[q] : 0.0;
   b->host = new (std::nothrow) fw_tissue_desc[max_tissues]();
-  if (!b->host || hipMalloc(&b->table, sizeof(BatchTissue) * max_tissues) != hipSuccess ||
+  if (!b->host || !b->active ||
+      hipMalloc(&b->table, sizeof(BatchTissue) * max_tissues) != hipSuccess ||
       hipMalloc(&b->step_words, 4 * sizeof(int32_t)) != hipSuccess) {
     set_error("fw_batch_create: allocation failed");
     (void)fw_batch_destroy(b);
@@ -575,6 +579,7 @@ int fw_batch_destroy(fw_batch* b) {
   if (b->bstart) (void)hipFree(b->bstart);
   if (b->step_words) (void)hipFree(b->step_words);
   delete[] b->host;
+  delete[] b->active;
   delete b;
   return FW_OK;
 }
@@ -662,21 +667,62 @@ int fw_batch_release(fw_batch* b, int32_t slot) {
   return FW_OK;
 }
 
-int fw_batch_block_nodes(int32_t model, int32_t precision) {
-  if (model < 0 || model >= FW_N_MODELS ||
-      (precision != FW_PREC_FP64 && precision != FW_PREC_FP32)) {
+int fw_batch_block_nodes(const fw_batch* b) {
+  if (!b) {
     set_error("fw_batch_block_nodes: bad arguments");
     return FW_E_INVALID;
   }
-  return batch_unroll(model, precision) * kBlock;
+  return batch_unroll(b->variant) * kBlock;
+}
+
+int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity) {
+  if (model < 0 || model >= FW_N_MODELS || (capacity > 0 && !variants)) {
+    set_error("fw_batch_variants: bad arguments");
+    return FW_E_INVALID;
+  }
+  int32_t n = 0;
+  for (int32_t v = 0; v < 4; ++v)
+    if (batch_variant_ok(model, v)) {
+      if (n < capacity) variants[n] = v;
+      ++n;
+    }
+  return n;
+}
+
+int fw_batch_get_variant(const fw_batch* b) {
+  if (!b) {
+    set_error("fw_batch_get_variant: bad arguments");
+    return FW_E_INVALID;
+  }
+  return b->variant;
+}
+
+int fw_batch_set_variant(fw_batch* b, int32_t variant, void* stream) {
+  if (!b || !batch_variant_ok(b->model, variant)) {
+    set_error("fw_batch_set_variant: variant %d not available for this batch", variant);
+    return FW_E_INVALID;
+  }
+  if (variant == b->variant) return FW_OK;
+  const int32_t old = b->variant;
+  b->variant = variant;
+  if (batch_unroll(old) != batch_unroll(variant) && b->n_active > 0) {
+    // the block table's granularity changes with the nodes per work-item
+    int rc = fw_batch_set_active(b, b->active, b->n_active, stream);
+    if (rc) {
+      b->variant = old;
+      (void)fw_batch_set_active(b, b->active, b->n_active, stream);
+      return rc;
+    }
+  }
+  return FW_OK;
 }
 
 int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void* stream) {
-  if (!b || n_slots < 0 || (n_slots && !slots)) {
+  if (!b || n_slots < 0 || n_slots > b->max_tissues || (n_slots && !slots)) {
     set_error("fw_batch_set_active: bad arguments");
     return FW_E_INVALID;
   }
-  const int64_t per = (int64_t)batch_unroll(b->model, b->precision) * kBlock;
+  const int64_t per = (int64_t)batch_unroll(b->variant) * kBlock;
   int64_t nblocks = 0;
   for (int32_t q = 0; q < n_slots; ++q) {
     const int32_t s = slots[q];
@@ -742,6 +788,9 @@ int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void
   }
   free(hb);
   b->nblocks = (int32_t)nblocks;
+  if (slots != b->active)
+    for (int32_t q = 0; q < n_slots; ++q) b->active[q] = slots[q];
+  b->n_active = n_slots;
   return FW_OK;
 }
 
@@ -755,7 +804,7 @@ int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph,
   hipStream_t s = as_stream(stream);
   int32_t done = 0;
   if (use_graph && n_steps >= 2) {
-    if (!b->graph || b->graph_nblocks != b->nblocks) {
+    if (!b->graph || b->graph_nblocks != b->nblocks || b->graph_variant != b->variant) {
       if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -783,6 +832,7 @@ int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph,
       }
       b->graph = ex;
       b->graph_nblocks = b->nblocks;
+      b->graph_variant = b->variant;
     }
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, b->step_words, (int32_t)step0);
     const int32_t pairs = n_steps / 2;

@@ -145,10 +145,24 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
   }
 }
 
-// Nodes per work-item: 4 consecutive ones for the light models, else 1; a
-// block covers batch_unroll * kBlock consecutive nodes of one tissue.
-__host__ __device__ constexpr int batch_unroll(int model, int precision) {
-  return model <= 4 ? 4 : 1;
+// Launch-shape variants of the batched step (chosen per batch, e.g. by the
+// autotuner of finitewave_amd.batch; all give the same results bit for bit):
+//   bit 0: 4 consecutive nodes per work-item (light models only), else 1;
+//   bit 1: XCD-contiguous block map (launched block b -> logical block
+//          (b mod 8) * per_xcd + b div 8), else launched block = logical block.
+// A block covers batch_unroll(variant) * kBlock consecutive nodes of a tissue.
+constexpr int kBatchVec = 1, kBatchXcd = 2;
+
+__host__ __device__ constexpr bool batch_variant_ok(int model, int variant) {
+  return variant >= 0 && variant < 4 && (model <= 4 || (variant & kBatchVec) == 0);
+}
+
+__host__ __device__ constexpr int batch_default_variant(int model) {
+  return model <= 4 ? (kBatchVec | kBatchXcd) : kBatchXcd;
+}
+
+__host__ __device__ constexpr int batch_unroll(int variant) {
+  return (variant & kBatchVec) ? 4 : 1;
 }
 
 // 4 values of consecutive nodes, element-aligned: one vector load or store
@@ -170,14 +184,14 @@ template <typename T>
 using Vec4 = typename Vec4T<T>::type;
 typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
 
-template <int MODEL, typename R>
+template <int MODEL, typename R, int KU, bool XCD>
 __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<MODEL, R>> a) {
   using U = UType<MODEL, R>;  // the potential's storage type
   constexpr bool EXACT = sizeof(R) == 8;
   constexpr int NS = model_n_state(MODEL);
-  constexpr int KU = batch_unroll(MODEL, sizeof(R) == 4 ? FW_PREC_FP32 : FW_PREC_FP64);
+  static_assert(KU == 1 || (KU == 4 && MODEL <= 4), "4 nodes per work-item: light models");
   const uint32_t b = blockIdx.x;
-  const uint32_t lb = (b % kXcds) * a.per_xcd + b / kXcds;
+  const uint32_t lb = XCD ? (b % kXcds) * a.per_xcd + b / kXcds : b;
   if (lb >= a.nblocks) return;
   // the record, read uniformly before any store
   const int32_t t = __builtin_amdgcn_readfirstlane(a.btis[lb]);

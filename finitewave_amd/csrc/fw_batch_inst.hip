@@ -38,8 +38,24 @@ int FW_BLAUNCHER(const BatchLaunch& L, hipStream_t s) {
   a.dt = (BReal)L.dt;
   a.s = (UReal)(L.D * L.dt / L.dr2);
   a.P = L.P;
-  hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal>), dim3(L.per_xcd * kXcds), dim3(kBlock), 0,
-                     s, a);
+  const dim3 gx(L.per_xcd * kXcds), g1(L.nblocks), blk(kBlock);
+  switch (L.variant) {
+    case 0: hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 1, false>), g1, blk, 0, s, a); break;
+    case kBatchXcd:
+      hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 1, true>), gx, blk, 0, s, a);
+      break;
+#if FW_MODEL <= 4
+    case kBatchVec:
+      hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 4, false>), g1, blk, 0, s, a);
+      break;
+    case kBatchVec | kBatchXcd:
+      hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 4, true>), gx, blk, 0, s, a);
+      break;
+#endif
+    default:
+      set_error("batch launch: variant %d not available for model %d", L.variant, FW_MODEL);
+      return FW_E_INVALID;
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     set_error("batch launch failed: %s", hipGetErrorString(e));

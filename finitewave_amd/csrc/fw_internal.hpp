@@ -87,6 +87,7 @@ struct BatchLaunch {
   const int32_t* step_in;
   int32_t* step_out;
   int32_t step;
+  int32_t variant;  // launch shape (fw_batch.hpp: kBatchVec | kBatchXcd)
   double D, dt, dr2, inv_dr2;
   Params P;
 };
@@ -106,6 +107,9 @@ struct fw_batch {
   int32_t nblocks, cap_blocks;
   int32_t* step_words;    // device int32[4]: graph step words [0..1], border flag [2]
   hipGraphExec_t graph;
-  int32_t graph_nblocks;
+  int32_t graph_nblocks, graph_variant;
+  int32_t variant;        // launch shape of fw_batch_run
+  int32_t* active;        // host: the occupied slots listed by fw_batch_set_active
+  int32_t n_active;
   fw_tissue_desc* host;   // host mirror of the table
 };

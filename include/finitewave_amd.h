@@ -296,8 +296,18 @@ int fw_batch_set_active(fw_batch* batch, const int32_t* slots, int32_t n_slots,
 /* Empties a slot: it can no longer be listed in fw_batch_set_active or
  * stimulated until fw_batch_set_tissue fills it again. */
 int fw_batch_release(fw_batch* batch, int32_t slot);
-/* Nodes per block of the batched step for (model, precision). */
-int fw_batch_block_nodes(int32_t model, int32_t precision);
+/* Nodes per block of the batch's current launch shape. */
+int fw_batch_block_nodes(const fw_batch* batch);
+/* Launch-shape variants of the batched step for a model (all give the same
+ * results bit for bit): bit 0 = 4 consecutive nodes per work-item (models
+ * 0-4), bit 1 = XCD-contiguous block map.  Writes up to `capacity` of them to
+ * `variants` and returns how many there are. */
+int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity);
+/* Selects the launch shape of fw_batch_run (rebuilds the block table when the
+ * nodes per work-item change); returns FW_E_INVALID for a variant the model
+ * does not have. */
+int fw_batch_set_variant(fw_batch* batch, int32_t variant, void* stream);
+int fw_batch_get_variant(const fw_batch* batch);
 /* n_steps global steps from step0; use_graph replays a captured two-step
  * graph (recaptured when the block count changes). */
 int fw_batch_run(fw_batch* batch, int64_t step0, int32_t n_steps, int32_t use_graph,

@@ -217,3 +217,47 @@ def test_batch_abi_rejects_updated_nodes_on_the_outer_layer(gpu):
         if not ok:
             assert b"outer" in _native.lib().fw_last_error()
     b.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,prec", [("mitchell_schaeffer", "fp32"), ("aliev_panfilov", "fp64"),
+                                        ("tp06", "fp32")])
+def test_every_launch_variant_gives_the_same_results(gpu, model, prec):
+    cls = CLS[model]
+    results = {}
+    probe = TissueBatch(cls, precision=prec, max_tissues=2)
+    variants = probe.variants()
+    probe.close()
+    for v in variants:
+        sims = _sims("tp06" if model == "tp06" else "aniso", seed=11)
+        sims[1].tissue.fibers = None  # iso and aniso tissues in one batch
+        b = TissueBatch(cls, precision=prec, max_tissues=4, autotune=False, chunk_cap=90)
+        b.set_variant(v)
+        for s in sims:
+            b.admit(s)
+        b.run_all()
+        assert b.variant == v
+        b.close()
+        results[v] = [s.result for s in sims]
+    ref = results[variants[0]]
+    for v in variants[1:]:
+        for a, r in zip(results[v], ref):
+            _check_same(a, r)
+
+
+@pytest.mark.gpu
+def test_autotuner_picks_a_variant_and_keeps_results(gpu, tmp_path, monkeypatch):
+    from finitewave_amd import batch as B
+    monkeypatch.setenv("FW_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(B, "_TUNE_CACHE", {})
+    sims = _sims("iso", seed=12, t_max=6.0)
+    b = TissueBatch(fw.MitchellSchaeffer2D, precision="fp32", max_tissues=4, tune_steps=8)
+    for s in sims:
+        b.admit(s)
+    b.run_all()
+    assert b.tuned and b.tuning is not None and set(b.tuning) == set(b.variants())
+    assert b.variant == min(b.tuning, key=b.tuning.get)
+    assert (tmp_path / "tune.json").exists()
+    b.close()
+    for s in sims:
+        _check_same(s.result, _single(fw.MitchellSchaeffer2D, "fp32", s))

@@ -347,3 +347,14 @@ def test_period_tracker_host_rule_equals_oracle():
     assert len(tr.act_t) == len(otr.data["act_t"])
     periods = tr.output
     assert len(periods) == len(cells)
+
+
+def test_batch_launch_variants_per_model():
+    import ctypes
+    L = _native.lib()
+    L.fw_batch_variants.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]
+    for model, expect in ((2, [0, 1, 2, 3]), (6, [0, 2]), (7, [0, 2])):
+        arr = (ctypes.c_int32 * 4)()
+        n = L.fw_batch_variants(model, arr, 4)
+        assert list(arr[:n]) == expect
+    assert L.fw_batch_variants(99, None, 0) == -1
