@@ -32,10 +32,9 @@ struct StatePtrs {
 };
 
 // ---- math dispatch --------------------------------------------------------
-// FP64: the table-driven exp/log of fw_math.hpp, correctly rounded except in
-// rare near-halfway cases, so the kernel's last bits follow the host libm the
-// reference's numba kernels call; FP32: the device library.
-__device__ __forceinline__ double fexp(double x) { return dd_exp(x); }
+// FP64: the exp/log of fw_math.hpp, whose last bits follow the host libm the
+// reference's numba kernels call (exp bit for bit); FP32: the device library.
+__device__ __forceinline__ double fexp(double x) { return t128_exp(x); }
 __device__ __forceinline__ float fexp(float x) { return __expf(x); }
 __device__ __forceinline__ double flog(double x) { return dd_log(x); }
 __device__ __forceinline__ float flog(float x) { return __logf(x); }
@@ -52,6 +51,25 @@ __device__ __forceinline__ float fpow(float x, float y) { return powf(x, y); }
 // literal form divides by zero there.
 __device__ __forceinline__ float exprel(float y) {
   return fabsf(y) < 1e-4f ? 1.0f + 0.5f * y : expm1f(y) / y;
+}
+
+// a / b for a literal b.  FP64 path: the correctly rounded quotient in five
+// operations instead of a full division, from y = 1/b correctly rounded
+// (folded at compile time): q0 = a y is within 1.5 ulp of a/b, one
+// correction q1 = q0 + (a - b q0) y is faithful, and a second gives RN(a/b)
+// by Markstein's theorem (y within half an ulp of 1/b, q1 within one ulp of
+// a/b; the remainder a - b q1 is exact).  Holds while a/b is a normal number
+// or zero (a zero quotient comes out +0).  FP32 path: the plain division.
+template <typename R>
+__device__ __forceinline__ R divc(R a, R b) {
+  if constexpr (sizeof(R) == 8) {
+    const R y = R(1) / b;
+    const R q0 = a * y;
+    const R q1 = fma(fma(-b, q0, a), y, q0);
+    return fma(fma(-b, q1, a), y, q1);
+  } else {
+    return a / b;
+  }
 }
 
 // state arrays live in global memory: accesses through the global address
@@ -380,23 +398,23 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     auto& mr = ref<R>(S, 5, c);
     auto& hr = ref<G>(S, 6, c);
     auto& jr = ref<G>(S, 7, c);
-    const R alpha_m = R(1.0) / (R(1.0) + fexp((R(-60.0) - u) / R(5.0)));
-    const R beta_m = R(0.1) / (R(1.0) + fexp((u + R(35.0)) / R(5.0))) +
-                     R(0.1) / (R(1.0) + fexp((u - R(50.0)) / R(200.0)));
+    const R alpha_m = R(1.0) / (R(1.0) + fexp(divc(R(-60.0) - u, R(5.0))));
+    const R beta_m = R(0.1) / (R(1.0) + fexp(divc(u + R(35.0), R(5.0)))) +
+                     R(0.1) / (R(1.0) + fexp(divc(u - R(50.0), R(200.0))));
     const R tau_m = alpha_m * beta_m;
-    const R m_inf = R(1.0) / ((R(1.0) + fexp((R(-56.86) - u) / R(9.03))) *
-                              (R(1.0) + fexp((R(-56.86) - u) / R(9.03))));
+    const R m_inf = R(1.0) / ((R(1.0) + fexp(divc(R(-56.86) - u, R(9.03)))) *
+                              (R(1.0) + fexp(divc(R(-56.86) - u, R(9.03)))));
     R alpha_h, beta_h;
     if (u >= R(-40.0)) {
       alpha_h = R(0.0);
-      beta_h = R(0.77) / (R(0.13) * (R(1.0) + fexp(-(u + R(10.66)) / R(11.1))));
+      beta_h = R(0.77) / (R(0.13) * (R(1.0) + fexp(divc(-(u + R(10.66)), R(11.1)))));
     } else {
-      alpha_h = R(0.057) * fexp(-(u + R(80.0)) / R(6.8));
+      alpha_h = R(0.057) * fexp(divc(-(u + R(80.0)), R(6.8)));
       beta_h = R(2.7) * fexp(R(0.079) * u) + R(310000.0) * fexp(R(0.3485) * u);
     }
     const R tau_h = R(1.0) / (alpha_h + beta_h);
-    const R h_inf = R(1.0) / ((R(1.0) + fexp((u + R(71.55)) / R(7.43))) *
-                              (R(1.0) + fexp((u + R(71.55)) / R(7.43))));
+    const R h_inf = R(1.0) / ((R(1.0) + fexp(divc(u + R(71.55), R(7.43)))) *
+                              (R(1.0) + fexp(divc(u + R(71.55), R(7.43)))));
     R alpha_j, beta_j;
     if (u >= R(-40.0)) {
       alpha_j = R(0.0);
@@ -427,23 +445,23 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     auto& fr = ref<R>(S, 14, c);
     auto& f2r = ref<R>(S, 15, c);
     auto& fcr = ref<R>(S, 16, c);
-    const R d_inf = R(1.0) / (R(1.0) + fexp((R(-8) - u) / R(7.5)));
-    const R Ad = R(1.4) / (R(1.0) + fexp((R(-35) - u) / R(13))) + R(0.25);
-    const R Bd = R(1.4) / (R(1.0) + fexp((u + R(5)) / R(5)));
-    const R Cd = R(1.0) / (R(1.0) + fexp((R(50) - u) / R(20)));
+    const R d_inf = R(1.0) / (R(1.0) + fexp(divc(R(-8) - u, R(7.5))));
+    const R Ad = R(1.4) / (R(1.0) + fexp(divc(R(-35) - u, R(13)))) + R(0.25);
+    const R Bd = R(1.4) / (R(1.0) + fexp(divc(u + R(5), R(5))));
+    const R Cd = R(1.0) / (R(1.0) + fexp(divc(R(50) - u, R(20))));
     const R tau_d = Ad * Bd + Cd;
-    const R f_inf = R(1.0) / (R(1.0) + fexp((u + R(20)) / R(7)));
-    const R Af = R(1102.5) * fexp(-(u + R(27)) * (u + R(27)) / R(225));
-    const R Bf = R(200.0) / (R(1) + fexp((R(13) - u) / R(10.0)));
-    const R Cf = R(180.0) / (R(1) + fexp((u + R(30)) / R(10))) + R(20);
+    const R f_inf = R(1.0) / (R(1.0) + fexp(divc(u + R(20), R(7))));
+    const R Af = R(1102.5) * fexp(divc(-(u + R(27)) * (u + R(27)), R(225)));
+    const R Bf = R(200.0) / (R(1) + fexp(divc(R(13) - u, R(10.0))));
+    const R Cf = R(180.0) / (R(1) + fexp(divc(u + R(30), R(10)))) + R(20);
     const R tau_f = Af + Bf + Cf;
-    const R f2_inf = R(0.67) / (R(1.0) + fexp((u + R(35)) / R(7))) + R(0.33);
-    const R Af2 = R(600) * fexp(-(u + R(25)) * (u + R(25)) / R(170));
-    const R Bf2 = R(31) / (R(1.0) + fexp((R(25) - u) / R(10)));
-    const R Cf2 = R(16) / (R(1.0) + fexp((u + R(30)) / R(10)));
+    const R f2_inf = R(0.67) / (R(1.0) + fexp(divc(u + R(35), R(7)))) + R(0.33);
+    const R Af2 = R(600) * fexp(divc(-(u + R(25)) * (u + R(25)), R(170)));
+    const R Bf2 = R(31) / (R(1.0) + fexp(divc(R(25) - u, R(10))));
+    const R Cf2 = R(16) / (R(1.0) + fexp(divc(u + R(30), R(10))));
     const R tau_f2 = Af2 + Bf2 + Cf2;
-    const R fcass_inf = R(0.6) / (R(1) + cass / R(0.05) * (cass / R(0.05))) + R(0.4);
-    const R tau_fcass = R(80.0) / (R(1) + cass / R(0.05) * (cass / R(0.05))) + R(2.0);
+    const R fcass_inf = R(0.6) / (R(1) + divc(cass, R(0.05)) * divc(cass, R(0.05))) + R(0.4);
+    const R tau_fcass = R(80.0) / (R(1) + divc(cass, R(0.05)) * divc(cass, R(0.05))) + R(2.0);
     const R d = d_inf - (d_inf - dr) * fexp(-dt / tau_d);
     const R f = f_inf - (f_inf - fr) * fexp(-dt / tau_f);
     const R f2 = f2_inf - (f2_inf - f2r) * fexp(-dt / tau_f2);
@@ -468,11 +486,11 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     auto& rr_ = ref<R>(S, 11, c);
     auto& sr = ref<R>(S, 12, c);
-    const R r_inf = R(1.0) / (R(1.0) + fexp((R(20) - u) / R(6.0)));
-    const R s_inf = R(1.0) / (R(1.0) + fexp((u + R(20)) / R(5.0)));
-    const R tau_r = R(9.5) * fexp(-(u + R(40.0)) * (u + R(40.0)) / R(1800.0)) + R(0.8);
-    const R tau_s = R(85.0) * fexp(-(u + R(45.0)) * (u + R(45.0)) / R(320.0)) +
-                    R(5.0) / (R(1.0) + fexp((u - R(20.0)) / R(5.0))) + R(3.0);
+    const R r_inf = R(1.0) / (R(1.0) + fexp(divc(R(20) - u, R(6.0))));
+    const R s_inf = R(1.0) / (R(1.0) + fexp(divc(u + R(20), R(5.0))));
+    const R tau_r = R(9.5) * fexp(divc(-(u + R(40.0)) * (u + R(40.0)), R(1800.0))) + R(0.8);
+    const R tau_s = R(85.0) * fexp(divc(-(u + R(45.0)) * (u + R(45.0)), R(320.0))) +
+                    R(5.0) / (R(1.0) + fexp(divc(u - R(20.0), R(5.0)))) + R(3.0);
     const R s = s_inf - (s_inf - sr) * fexp(-dt / tau_s);
     const R r = r_inf - (r_inf - rr_) * fexp(-dt / tau_r);
     sr = s;
@@ -484,13 +502,13 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     auto& x1r = ref<R>(S, 8, c);
     auto& x2r = ref<R>(S, 9, c);
-    const R xr1_inf = R(1.0) / (R(1.0) + fexp((R(-26.0) - u) / R(7.0)));
-    const R axr1 = R(450.0) / (R(1.0) + fexp((R(-45.0) - u) / R(10.0)));
-    const R bxr1 = R(6.0) / (R(1.0) + fexp((u - R(-30.0)) / R(11.5)));
+    const R xr1_inf = R(1.0) / (R(1.0) + fexp(divc(R(-26.0) - u, R(7.0))));
+    const R axr1 = R(450.0) / (R(1.0) + fexp(divc(R(-45.0) - u, R(10.0))));
+    const R bxr1 = R(6.0) / (R(1.0) + fexp(divc(u - R(-30.0), R(11.5))));
     const R tau_xr1 = axr1 * bxr1;
-    const R xr2_inf = R(1.0) / (R(1.0) + fexp((u - R(-88.0)) / R(24.0)));
-    const R axr2 = R(3.0) / (R(1.0) + fexp((R(-60.0) - u) / R(20.0)));
-    const R bxr2 = R(1.12) / (R(1.0) + fexp((u - R(60.0)) / R(20.0)));
+    const R xr2_inf = R(1.0) / (R(1.0) + fexp(divc(u - R(-88.0), R(24.0))));
+    const R axr2 = R(3.0) / (R(1.0) + fexp(divc(R(-60.0) - u, R(20.0))));
+    const R bxr2 = R(1.12) / (R(1.0) + fexp(divc(u - R(60.0), R(20.0))));
     const R tau_xr2 = axr2 * bxr2;
     const R xr1 = xr1_inf - (xr1_inf - x1r) * fexp(-dt / tau_xr1);
     const R xr2 = xr2_inf - (xr2_inf - x2r) * fexp(-dt / tau_xr2);
@@ -502,10 +520,10 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   R iks;
   {
     auto& xsr = ref<R>(S, 10, c);
-    const R Axs = R(1400.0) / fsqrt(R(1.0) + fexp((R(5.0) - u) / R(6)));
-    const R Bxs = R(1.0) / (R(1.0) + fexp((u - R(35.0)) / R(15.0)));
+    const R Axs = R(1400.0) / fsqrt(R(1.0) + fexp(divc(R(5.0) - u, R(6))));
+    const R Bxs = R(1.0) / (R(1.0) + fexp(divc(u - R(35.0), R(15.0))));
     const R tau_xs = Axs * Bxs + R(80);
-    const R xs_inf = R(1.0) / (R(1.0) + fexp((R(-5.0) - u) / R(14.0)));
+    const R xs_inf = R(1.0) / (R(1.0) + fexp(divc(R(-5.0) - u, R(14.0))));
     const R xs = xs_inf - (xs_inf - xsr) * fexp(-dt / tau_xs);
     xsr = xs;
     iks = gks * xs * xs * (u - Eks);
@@ -529,7 +547,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
                                R(0.0353) * fexp(-u * F / (R_ * T)));
   const R inak = knak * (ko / (ko + KmK)) * (nai / (nai + KmNa)) * rec_iNaK;
   const R ipca = gpca * cai / (KpCa + cai);
-  const R rec_ipK = R(1.0) / (R(1.0) + fexp((R(25) - u) / R(5.98)));
+  const R rec_ipK = R(1.0) / (R(1.0) + fexp(divc(R(25) - u, R(5.98))));
   const R ipk = gpk * rec_ipK * (u - Ek);
   const R ibna = gbna * (u - Ena);
   const R ibca = gbca * (u - Eca);
