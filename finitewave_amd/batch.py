@@ -117,6 +117,7 @@ def stim_schedule(stims, n_steps: int, fold: np.ndarray) -> list:
     return [(k, st) for k, _, st in ev]
 
 
+_XCD_BIT = 2  # launch-shape bit of the XCD-contiguous block map (fw_batch.hpp)
 _TUNE_CACHE: dict = {}
 
 
@@ -392,12 +393,13 @@ class TissueBatch:
             timings[v] = e0.elapsed_time(e1) / self.tune_steps
             step += per
         best = min(timings, key=timings.get)
-        # near-ties keep the default shape: its XCD-contiguous block map reads
-        # fewer bytes from HBM (neighbour rows stay in one XCD's L2), which
-        # step time alone does not show when the Infinity Cache hides them
-        default = vs[-1]  # variants are listed in increasing order; the last has both bits
-        if timings.get(default, math.inf) <= 1.03 * timings[best]:
-            best = default
+        # among near-ties (within 3 %) prefer the XCD-contiguous block map: it
+        # reads fewer bytes from HBM (neighbour rows stay in one XCD's L2),
+        # which step time alone does not show when the Infinity Cache hides
+        # the re-reads
+        near = [v for v in vs if timings[v] <= 1.03 * timings[best] and v & _XCD_BIT]
+        if near and not best & _XCD_BIT:
+            best = min(near, key=timings.get)
         self.set_variant(best)
         self.tuning = timings
         self.tuned = True

@@ -681,7 +681,7 @@ int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity) {
     return FW_E_INVALID;
   }
   int32_t n = 0;
-  for (int32_t v = 0; v < 4; ++v)
+  for (int32_t v = 0; v < kBatchVariants; ++v)
     if (batch_variant_ok(model, v)) {
       if (n < capacity) variants[n] = v;
       ++n;

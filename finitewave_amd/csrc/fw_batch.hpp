@@ -149,16 +149,18 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
 // autotuner of finitewave_amd.batch; all give the same results bit for bit):
 //   bit 0: 4 consecutive nodes per work-item (light models only), else 1;
 //   bit 1: XCD-contiguous block map (launched block b -> logical block
-//          (b mod 8) * per_xcd + b div 8), else launched block = logical block.
+//          (b mod 8) * per_xcd + b div 8), else launched block = logical block;
+//   bit 2: state and new potential read and written with the non-temporal
+//          hint (faster once a step's data is well beyond the Infinity Cache).
 // A block covers batch_unroll(variant) * kBlock consecutive nodes of a tissue.
-constexpr int kBatchVec = 1, kBatchXcd = 2;
+constexpr int kBatchVec = 1, kBatchXcd = 2, kBatchNT = 4, kBatchVariants = 8;
 
 __host__ __device__ constexpr bool batch_variant_ok(int model, int variant) {
-  return variant >= 0 && variant < 4 && (model <= 4 || (variant & kBatchVec) == 0);
+  return variant >= 0 && variant < kBatchVariants && (model <= 4 || (variant & kBatchVec) == 0);
 }
 
 __host__ __device__ constexpr int batch_default_variant(int model) {
-  return model <= 4 ? (kBatchVec | kBatchXcd) : kBatchXcd;
+  return model <= 4 ? (kBatchVec | kBatchXcd | kBatchNT) : (kBatchXcd | kBatchNT);
 }
 
 __host__ __device__ constexpr int batch_unroll(int variant) {
@@ -184,9 +186,12 @@ template <typename T>
 using Vec4 = typename Vec4T<T>::type;
 typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
 
-template <int MODEL, typename R, int KU, bool XCD>
+template <int MODEL, typename R, int V>
 __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<MODEL, R>> a) {
   using U = UType<MODEL, R>;  // the potential's storage type
+  constexpr int KU = batch_unroll(V);
+  constexpr bool XCD = (V & kBatchXcd) != 0;
+  constexpr bool NT = (V & kBatchNT) != 0;
   constexpr bool EXACT = sizeof(R) == 8;
   constexpr int NS = model_n_state(MODEL);
   static_assert(KU == 1 || (KU == 4 && MODEL <= 4), "4 nodes per work-item: light models");
@@ -221,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
     dd[q] = d.dd[q];
     df[q] = d.df[q];
   }
-  StatePtrs S;
+  std::conditional_t<NT, StreamStatePtrs, StatePtrs> S;
 #pragma unroll
   for (int k = 0; k < NS; ++k) S.p[k] = d.state[k];
 
@@ -252,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
       }
     U val = diffuse9<U, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
     val = ionic_update<MODEL, R>(uc, val, S, (int64_t)c, a.dt, a.P);
-    un[c] = val;
+    st_state<NT>(un + c, val);
   } else {
     const int32_t c = start + 4 * (int32_t)threadIdx.x;  // a multiple of 4
     if (c >= n) return;
@@ -296,7 +301,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
       }
       Vec4<R> st[NS > 0 ? NS : 1];
 #pragma unroll
-      for (int s = 0; s < NS; ++s) st[s] = *(const global_t<Vec4<R>>*)((global_t<R>*)S.p[s] + c);
+      for (int s = 0; s < NS; ++s)
+        st[s] = ld_state<NT>((const global_t<Vec4<R>>*)((global_t<R>*)S.p[s] + c));
       Vec4<R> out;
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
@@ -319,9 +325,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
 #pragma unroll
         for (int s = 0; s < NS; ++s) st[s][k] = upd ? rs.v[s] : st[s][k];
       }
-      *(global_t<Vec4<R>>*)(un + c) = out;
+      st_state<NT>((global_t<Vec4<R>>*)(un + c), out);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) *(global_t<Vec4<R>>*)((global_t<R>*)S.p[s] + c) = st[s];
+      for (int s = 0; s < NS; ++s) st_state<NT>((global_t<Vec4<R>>*)((global_t<R>*)S.p[s] + c), st[s]);
       return;
     }
     // first and last rows and the tissue's tail: node by node
@@ -346,12 +352,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
         }
       RegState<R, NS> rs;
 #pragma unroll
-      for (int s = 0; s < NS; ++s) rs.v[s] = ((const global_t<R>*)S.p[s])[cc];
+      for (int s = 0; s < NS; ++s) rs.v[s] = ld_state<NT>((const global_t<R>*)S.p[s] + cc);
       R val = diffuse9<R, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
       val = ionic_update<MODEL, R>(uc, val, rs, 0, a.dt, a.P);
-      un[cc] = val;
+      st_state<NT>(un + cc, val);
 #pragma unroll
-      for (int s = 0; s < NS; ++s) ((global_t<R>*)S.p[s])[cc] = rs.v[s];
+      for (int s = 0; s < NS; ++s) st_state<NT>((global_t<R>*)S.p[s] + cc, rs.v[s]);
     }
   }
 }

@@ -40,18 +40,22 @@ int FW_BLAUNCHER(const BatchLaunch& L, hipStream_t s) {
   a.P = L.P;
   const dim3 gx(L.per_xcd * kXcds), g1(L.nblocks), blk(kBlock);
   switch (L.variant) {
-    case 0: hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 1, false>), g1, blk, 0, s, a); break;
-    case kBatchXcd:
-      hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 1, true>), gx, blk, 0, s, a);
-      break;
+#define FW_BCASE(V)                                                                          \
+  case V:                                                                                    \
+    hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, V>), ((V)&kBatchXcd) ? gx : g1, blk, 0, \
+                       s, a);                                                                \
+    break;
+    FW_BCASE(0)
+    FW_BCASE(kBatchXcd)
+    FW_BCASE(kBatchNT)
+    FW_BCASE(kBatchXcd | kBatchNT)
 #if FW_MODEL <= 4
-    case kBatchVec:
-      hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 4, false>), g1, blk, 0, s, a);
-      break;
-    case kBatchVec | kBatchXcd:
-      hipLaunchKernelGGL((k_batch_step<FW_MODEL, BReal, 4, true>), gx, blk, 0, s, a);
-      break;
+    FW_BCASE(kBatchVec)
+    FW_BCASE(kBatchVec | kBatchXcd)
+    FW_BCASE(kBatchVec | kBatchNT)
+    FW_BCASE(kBatchVec | kBatchXcd | kBatchNT)
 #endif
+#undef FW_BCASE
     default:
       set_error("batch launch: variant %d not available for model %d", L.variant, FW_MODEL);
       return FW_E_INVALID;

@@ -31,6 +31,12 @@ struct StatePtrs {
   void* p[24];
 };
 
+// The same pointers, for kernels that stream the state: model code reading
+// and writing through these gets non-temporal accesses (StateRef below).
+struct StreamStatePtrs {
+  void* p[24];
+};
+
 // ---- math dispatch --------------------------------------------------------
 // FP64: the exp/log of fw_math.hpp, whose last bits follow the host libm the
 // reference's numba kernels call (exp bit for bit); FP32: the device library.
@@ -82,6 +88,40 @@ __device__ __forceinline__ global_t<T>& ref(const StatePtrs& S, int k, int64_t c
   return ((global_t<T>*)S.p[k])[c];
 }
 
+// A node's state value in global memory, accessed with the non-temporal hint.
+// Each value is read once and written once per step, so when a step's data is
+// much larger than the caches, streaming it past them reaches a higher share
+// of HBM bandwidth; when it nearly fits the 256 MB Infinity Cache the default
+// policy is faster (it keeps part of the state there between steps).  Model
+// code binds a reference with auto&&, reads it into a local and assigns the
+// result back, so the same code serves both kinds of pointers.
+template <typename T>
+struct StateRef {
+  global_t<T>* p;
+  __device__ __forceinline__ operator T() const { return __builtin_nontemporal_load(p); }
+  __device__ __forceinline__ const StateRef& operator=(T v) const {
+    __builtin_nontemporal_store(v, p);
+    return *this;
+  }
+};
+
+template <typename T>
+__device__ __forceinline__ StateRef<T> ref(const StreamStatePtrs& S, int k, int64_t c) {
+  return StateRef<T>{(global_t<T>*)S.p[k] + c};
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_state(const global_t<T>* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+template <bool NT, typename T>
+__device__ __forceinline__ void st_state(global_t<T>* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // The state of one node held in registers (models whose state is all of the
 // arithmetic type): the same update code then reads and writes registers,
 // so a kernel can load the state of several nodes before storing any.
@@ -119,7 +159,7 @@ __device__ __forceinline__ R ionic_ap(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
   const R a = (R)P.v[0], k = (R)P.v[1], eap = (R)P.v[2], mu_1 = (R)P.v[3],
           mu_2 = (R)P.v[4];
-  auto& vref = ref<R>(S, 0, c);
+  auto&& vref = ref<R>(S, 0, c);
   R v = vref;
   // calc_v: v += (-dt*(eap + (mu_1*v)/(mu_2 + u)) * (v + k*u*(u - a - 1.)))
   v += (-dt * (eap + (mu_1 * v) / (mu_2 + u)) * (v + k * u * (u - a - R(1))));
@@ -133,7 +173,7 @@ template <typename R, typename SP>
 __device__ __forceinline__ R ionic_barkley(R u, R un, SP& S,
                                            int64_t c, R dt, const Params& P) {
   const R a = (R)P.v[0], b = (R)P.v[1], eap = (R)P.v[2];
-  auto& vref = ref<R>(S, 0, c);
+  auto&& vref = ref<R>(S, 0, c);
   R v = vref;
   v += dt * (u - v);
   vref = v;
@@ -146,7 +186,7 @@ __device__ __forceinline__ R ionic_ms(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
   const R tau_close = (R)P.v[0], tau_open = (R)P.v[1], tau_in = (R)P.v[2],
           tau_out = (R)P.v[3], u_gate = (R)P.v[4];
-  auto& href = ref<R>(S, 0, c);
+  auto&& href = ref<R>(S, 0, c);
   R h = href;
   h += (u < u_gate) ? dt * (R(1) - h) / tau_open : dt * (-h) / tau_close;
   href = h;
@@ -164,8 +204,8 @@ __device__ __forceinline__ R ionic_fk(R u, R un, SP& S, int64_t c,
           tau_si = (R)P.v[3], tau_v_m = (R)P.v[4], tau_v_p = (R)P.v[5],
           tau_w_m = (R)P.v[6], tau_w_p = (R)P.v[7], k = (R)P.v[8],
           u_c = (R)P.v[9], uc_si = (R)P.v[10];
-  auto& vref = ref<R>(S, 0, c);
-  auto& wref = ref<R>(S, 1, c);
+  auto&& vref = ref<R>(S, 0, c);
+  auto&& wref = ref<R>(S, 1, c);
   R v = vref, w = wref;
   const R H1 = (u_c - u) >= 0 ? R(1) : R(0);
   const R H2 = (u - u_c) >= 0 ? R(1) : R(0);
@@ -194,9 +234,9 @@ __device__ __forceinline__ R ionic_bo(R u, R un, SP& S, int64_t c,
           tau_s1 = (R)P.v[21], tau_s2 = (R)P.v[22], k_s = (R)P.v[23],
           u_s = (R)P.v[24], tau_si = (R)P.v[25], tau_w_inf = (R)P.v[26],
           w_inf_ = (R)P.v[27];
-  auto& vref = ref<R>(S, 0, c);
-  auto& wref = ref<R>(S, 1, c);
-  auto& sref = ref<R>(S, 2, c);
+  auto&& vref = ref<R>(S, 0, c);
+  auto&& wref = ref<R>(S, 1, c);
+  auto&& sref = ref<R>(S, 2, c);
   R v = vref, w = wref, s = sref;
   // calc_v(v, u, dt, theta_v, calc_v_inf(u), calc_tau_v_m(u), tau_v_p)
   {
@@ -251,13 +291,13 @@ __device__ __forceinline__ U ionic_lr91(U u_in, U un, SP& S, int64_t c, R dt,
           F = (R)P.v[13], PR_NaK = (R)P.v[14];
   // E_Na = R*T/F*np.log(nao/nai) (computed once per launch in the reference)
   const R E_Na = R_ * T / F * flog(nao / nai);
-  auto& mr = ref<R>(S, 0, c);
-  auto& hr = ref<R>(S, 1, c);
-  auto& jr = ref<R>(S, 2, c);
-  auto& dr = ref<R>(S, 3, c);
-  auto& fr = ref<R>(S, 4, c);
-  auto& xr = ref<R>(S, 5, c);
-  auto& cair = ref<R>(S, 6, c);
+  auto&& mr = ref<R>(S, 0, c);
+  auto&& hr = ref<R>(S, 1, c);
+  auto&& jr = ref<R>(S, 2, c);
+  auto&& dr = ref<R>(S, 3, c);
+  auto&& fr = ref<R>(S, 4, c);
+  auto&& xr = ref<R>(S, 5, c);
+  auto&& cair = ref<R>(S, 6, c);
   // calc_ina
   R ina;
   {
@@ -282,9 +322,9 @@ __device__ __forceinline__ U ionic_lr91(U u_in, U un, SP& S, int64_t c, R dt,
     else
       alpha_m = R(0.32) * (u + R(47.13)) / (R(1) - fexp(R(-0.1) * (u + R(47.13))));
     const R beta_m = R(0.08) * fexp(-u / R(11));
-    const R m = lr91_gate(mr, dt, alpha_m, beta_m);
-    const R h = lr91_gate(hr, dt, alpha_h, beta_h);
-    const R j = lr91_gate(jr, dt, alpha_J, beta_J);
+    const R m = lr91_gate<R>(mr, dt, alpha_m, beta_m);
+    const R h = lr91_gate<R>(hr, dt, alpha_h, beta_h);
+    const R j = lr91_gate<R>(jr, dt, alpha_J, beta_J);
     mr = m;
     hr = h;
     jr = j;
@@ -304,8 +344,8 @@ __device__ __forceinline__ U ionic_lr91(U u_in, U un, SP& S, int64_t c, R dt,
                       (R(1) + fexp(R(0.15) * (u + R(28))));
     const R beta_f = R(0.0065) * fexp(R(-0.02) * (u + R(30))) /
                      (R(1) + fexp(R(-0.2) * (u + R(30))));
-    d = lr91_gate(d, dt, alpha_d, beta_d);
-    f = lr91_gate(f, dt, alpha_f, beta_f);
+    d = lr91_gate<R>(d, dt, alpha_d, beta_d);
+    f = lr91_gate<R>(f, dt, alpha_f, beta_f);
     cai += dt * (R(-0.0001) * isi + R(0.07) * (R(0.0001) - cai));
     dr = d;
     fr = f;
@@ -332,7 +372,7 @@ __device__ __forceinline__ U ionic_lr91(U u_in, U un, SP& S, int64_t c, R dt,
                       (R(1) + fexp(R(0.057) * (u + R(50))));
     const R beta_x = R(0.0013) * fexp(R(-0.06) * (u + R(20))) /
                      (R(1) + fexp(R(-0.04) * (u + R(20))));
-    xr = lr91_gate(x, dt, alpha_x, beta_x);
+    xr = lr91_gate<R>(x, dt, alpha_x, beta_x);
   }
   const R E_K1 = R_ * T / F * flog(ko / ki);
   // calc_ik1
@@ -377,11 +417,11 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   const D inverseVcF = 1.0 / (Vc * Fd);
   const D inversevssF2 = 1.0 / (2 * Vss * Fd);
 
-  auto& cair = ref<D>(S, 0, c);
-  auto& casrr = ref<D>(S, 1, c);
-  auto& cassr = ref<D>(S, 2, c);
-  auto& nair = ref<D>(S, 3, c);
-  auto& Kir = ref<D>(S, 4, c);
+  auto&& cair = ref<D>(S, 0, c);
+  auto&& casrr = ref<D>(S, 1, c);
+  auto&& cassr = ref<D>(S, 2, c);
+  auto&& nair = ref<D>(S, 3, c);
+  auto&& Kir = ref<D>(S, 4, c);
   const D caiD = cair, casrD = casrr, cassD = cassr, naiD = nair, KiD = Kir;
   const R cai = (R)caiD, cass = (R)cassD, nai = (R)naiD, Ki = (R)KiD;
 
@@ -395,9 +435,9 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     // h, j: stored double on the FP32 path as well (state_is_f64)
     typedef std::conditional_t<sizeof(R) == 4, double, R> G;
-    auto& mr = ref<R>(S, 5, c);
-    auto& hr = ref<G>(S, 6, c);
-    auto& jr = ref<G>(S, 7, c);
+    auto&& mr = ref<R>(S, 5, c);
+    auto&& hr = ref<G>(S, 6, c);
+    auto&& jr = ref<G>(S, 7, c);
     const R alpha_m = R(1.0) / (R(1.0) + fexp(divc(R(-60.0) - u, R(5.0))));
     const R beta_m = R(0.1) / (R(1.0) + fexp(divc(u + R(35.0), R(5.0)))) +
                      R(0.1) / (R(1.0) + fexp(divc(u - R(50.0), R(200.0))));
@@ -441,10 +481,10 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_ical
   R ical;
   {
-    auto& dr = ref<R>(S, 13, c);
-    auto& fr = ref<R>(S, 14, c);
-    auto& f2r = ref<R>(S, 15, c);
-    auto& fcr = ref<R>(S, 16, c);
+    auto&& dr = ref<R>(S, 13, c);
+    auto&& fr = ref<R>(S, 14, c);
+    auto&& f2r = ref<R>(S, 15, c);
+    auto&& fcr = ref<R>(S, 16, c);
     const R d_inf = R(1.0) / (R(1.0) + fexp(divc(R(-8) - u, R(7.5))));
     const R Ad = R(1.4) / (R(1.0) + fexp(divc(R(-35) - u, R(13)))) + R(0.25);
     const R Bd = R(1.4) / (R(1.0) + fexp(divc(u + R(5), R(5))));
@@ -484,8 +524,8 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_ito
   R ito;
   {
-    auto& rr_ = ref<R>(S, 11, c);
-    auto& sr = ref<R>(S, 12, c);
+    auto&& rr_ = ref<R>(S, 11, c);
+    auto&& sr = ref<R>(S, 12, c);
     const R r_inf = R(1.0) / (R(1.0) + fexp(divc(R(20) - u, R(6.0))));
     const R s_inf = R(1.0) / (R(1.0) + fexp(divc(u + R(20), R(5.0))));
     const R tau_r = R(9.5) * fexp(divc(-(u + R(40.0)) * (u + R(40.0)), R(1800.0))) + R(0.8);
@@ -500,8 +540,8 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_ikr
   R ikr;
   {
-    auto& x1r = ref<R>(S, 8, c);
-    auto& x2r = ref<R>(S, 9, c);
+    auto&& x1r = ref<R>(S, 8, c);
+    auto&& x2r = ref<R>(S, 9, c);
     const R xr1_inf = R(1.0) / (R(1.0) + fexp(divc(R(-26.0) - u, R(7.0))));
     const R axr1 = R(450.0) / (R(1.0) + fexp(divc(R(-45.0) - u, R(10.0))));
     const R bxr1 = R(6.0) / (R(1.0) + fexp(divc(u - R(-30.0), R(11.5))));
@@ -519,7 +559,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_iks
   R iks;
   {
-    auto& xsr = ref<R>(S, 10, c);
+    auto&& xsr = ref<R>(S, 10, c);
     const R Axs = R(1400.0) / fsqrt(R(1.0) + fexp(divc(R(5.0) - u, R(6))));
     const R Bxs = R(1.0) / (R(1.0) + fexp(divc(u - R(35.0), R(15.0))));
     const R tau_xs = Axs * Bxs + R(80);
@@ -554,8 +594,8 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_irel (double: feeds the SR/SS calcium balance)
   D irel;
   {
-    auto& rrr = ref<R>(S, 17, c);
-    auto& oor = ref<R>(S, 18, c);
+    auto&& rrr = ref<R>(S, 17, c);
+    auto&& oor = ref<R>(S, 18, c);
     const D kCaSR = maxsr - (maxsr - minsr) / (1 + EC / casrD * (EC / casrD));
     const D k1 = k1_ / kCaSR;
     const D k2 = k2_ * kCaSR;
@@ -627,11 +667,11 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
           trpnmax = (R)P.v[30], cmdnmax = (R)P.v[31], csqnmax = (R)P.v[32],
           inacamax = (R)P.v[33], inakmax = (R)P.v[34], ipcamax = (R)P.v[35],
           krel = (R)P.v[36], iupmax = (R)P.v[37], kq10 = (R)P.v[38];
-  auto& nair = ref<R>(S, 0, c);
-  auto& kir = ref<R>(S, 1, c);
-  auto& cair = ref<R>(S, 2, c);
-  auto& caupr = ref<R>(S, 3, c);
-  auto& carelr = ref<R>(S, 4, c);
+  auto&& nair = ref<R>(S, 0, c);
+  auto&& kir = ref<R>(S, 1, c);
+  auto&& cair = ref<R>(S, 2, c);
+  auto&& caupr = ref<R>(S, 3, c);
+  auto&& carelr = ref<R>(S, 4, c);
   const R nai = nair, ki = kir, cai = cair, caup = caupr, carel = carelr;
   // calc_equilibrum_potentials
   const R ena = R_ * T / F * flog(nao / nai);
@@ -640,7 +680,7 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_gating_m / h / j
   R m, h, j;
   {
-    auto& mr = ref<R>(S, 5, c);
+    auto&& mr = ref<R>(S, 5, c);
     R am;
     if constexpr (sizeof(R) == 4) {
       am = R(3.2) / exprel(R(-0.1) * (u + R(47.13)));
@@ -654,11 +694,11 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
     const R bm = R(0.08) * fexp(-u / R(11));
     const R m_inf = am / (am + bm);
     const R tau_m = R(1) / (am + bm);
-    m = crn_gate(mr, m_inf, tau_m, dt);
+    m = crn_gate<R>(mr, m_inf, tau_m, dt);
     mr = m;
   }
   {
-    auto& hr = ref<R>(S, 6, c);
+    auto&& hr = ref<R>(S, 6, c);
     R ah, bh;
     if (u >= R(-40)) {
       ah = R(0);
@@ -669,11 +709,11 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
     }
     const R h_inf = ah / (ah + bh);
     const R tau_h = R(1) / (ah + bh);
-    h = crn_gate(hr, h_inf, tau_h, dt);
+    h = crn_gate<R>(hr, h_inf, tau_h, dt);
     hr = h;
   }
   {
-    auto& jr = ref<R>(S, 7, c);
+    auto&& jr = ref<R>(S, 7, c);
     R aj, bj;
     if (u >= R(-40)) {
       aj = R(0);
@@ -693,8 +733,8 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_ito
   R ito;
   {
-    auto& oar = ref<R>(S, 10, c);
-    auto& oir = ref<R>(S, 11, c);
+    auto&& oar = ref<R>(S, 10, c);
+    auto&& oir = ref<R>(S, 11, c);
     const R ao = R(0.65) / (fexp(-(u + R(10)) / R(8.5)) + fexp(-(u - R(30)) / R(59.0)));
     const R bo = R(0.65) / (R(2.5) + fexp((u + R(82)) / R(17.0)));
     const R tau_o = R(1) / (kq10 * (ao + bo));
@@ -703,8 +743,8 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
     const R boi = R(1) / (R(35.56) + fexp(-(u + R(1.26)) / R(7.44)));
     const R tau_oi = R(1) / (kq10 * (aoi + boi));
     const R oi_inf = R(1) / (R(1) + fexp((u + R(43.1)) / R(5.3)));
-    const R oa = crn_gate(oar, o_inf, tau_o, dt);
-    const R oi = crn_gate(oir, oi_inf, tau_oi, dt);
+    const R oa = crn_gate<R>(oar, o_inf, tau_o, dt);
+    const R oi = crn_gate<R>(oir, oi_inf, tau_oi, dt);
     oar = oa;
     oir = oi;
     ito = gto * (oa * (oa * oa)) * oi * (u - ek);
@@ -712,8 +752,8 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_ikur
   R ikur;
   {
-    auto& uar = ref<R>(S, 12, c);
-    auto& uir = ref<R>(S, 13, c);
+    auto&& uar = ref<R>(S, 12, c);
+    auto&& uir = ref<R>(S, 13, c);
     const R gkur = R(0.005) + R(0.05) / (R(1) + fexp(-(u - R(15)) / R(13.0)));
     const R aua = R(0.65) / (fexp(-(u + R(10)) / R(8.5)) + fexp(-(u - R(30)) / R(59.0)));
     const R bua = R(0.65) / (R(2.5) + fexp((u + R(82)) / R(17.0)));
@@ -723,8 +763,8 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
     const R bui = fexp((u - R(158)) / R(16.0));
     const R tau_ui = R(1) / (kq10 * (aui + bui));
     const R ui_inf = R(1) / (R(1) + fexp((u - R(99.45)) / R(27.48)));
-    const R ua = crn_gate(uar, ua_inf, tau_ua, dt);
-    const R ui = crn_gate(uir, ui_inf, tau_ui, dt);
+    const R ua = crn_gate<R>(uar, ua_inf, tau_ua, dt);
+    const R ui = crn_gate<R>(uir, ui_inf, tau_ui, dt);
     uar = ua;
     uir = ui;
     ikur = gkur_coeff * gkur * (ua * (ua * ua)) * ui * (u - ek);
@@ -732,7 +772,7 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_ikr on the kernel's xr slot (index 15), gkr hard-coded to 0.0294
   R ikr;
   {
-    auto& xrr = ref<R>(S, 15, c);
+    auto&& xrr = ref<R>(S, 15, c);
     const R gkr = R(0.0294);
     R axr, bxr;
     if constexpr (sizeof(R) == 4) {
@@ -744,14 +784,14 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
     }
     const R tau_xr = R(1) / (axr + bxr);
     const R xr_inf = R(1) / (R(1) + fexp(-(u + R(14.1)) / R(6.5)));
-    const R xr = crn_gate(xrr, xr_inf, tau_xr, dt);
+    const R xr = crn_gate<R>(xrr, xr_inf, tau_xr, dt);
     xrr = xr;
     ikr = gkr * xr * (u - ek) / (R(1) + fexp((u + R(15)) / R(22.4)));
   }
   // calc_iks on the kernel's xs slot (index 14)
   R iks;
   {
-    auto& xsr = ref<R>(S, 14, c);
+    auto&& xsr = ref<R>(S, 14, c);
     R axs, bxs;
     if constexpr (sizeof(R) == 4) {
       axs = R(4e-05 * 17) / exprel(-(u - R(19.9)) / R(17));
@@ -762,16 +802,16 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
     }
     const R tau_xs = R(1) / (R(2) * (axs + bxs));
     const R xs_inf = R(1) / fsqrt(R(1) + fexp(-(u - R(19.9)) / R(12.7)));
-    const R xs = crn_gate(xsr, xs_inf, tau_xs, dt);
+    const R xs = crn_gate<R>(xsr, xs_inf, tau_xs, dt);
     xsr = xs;
     iks = gks * (xs * xs) * (u - ek);
   }
   // calc_ical
   R ical;
   {
-    auto& dr = ref<R>(S, 8, c);
-    auto& fr = ref<R>(S, 9, c);
-    auto& fcar = ref<R>(S, 16, c);
+    auto&& dr = ref<R>(S, 8, c);
+    auto&& fr = ref<R>(S, 9, c);
+    auto&& fcar = ref<R>(S, 16, c);
     R tau_d;
     if constexpr (sizeof(R) == 4) {
       const R y = -(u + R(10)) / R(6.24);
@@ -786,9 +826,9 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
     const R f_inf = R(1) / (R(1) + fexp((u + R(28)) / R(6.9)));
     const R tau_fca = R(2);
     const R fca_inf = R(1) / (R(1) + cai / R(0.00035));
-    const R d = crn_gate(dr, d_inf, tau_d, dt);
-    const R f = crn_gate(fr, f_inf, tau_f, dt);
-    const R fca = crn_gate(fcar, fca_inf, tau_fca, dt);
+    const R d = crn_gate<R>(dr, d_inf, tau_d, dt);
+    const R f = crn_gate<R>(fr, f_inf, tau_f, dt);
+    const R fca = crn_gate<R>(fcar, fca_inf, tau_fca, dt);
     dr = d;
     fr = f;
     fcar = fca;
@@ -817,10 +857,10 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_irel (uses the irel stored by the previous step)
   R irel;
   {
-    auto& irr = ref<R>(S, 17, c);
-    auto& vrr = ref<R>(S, 18, c);
-    auto& urr = ref<R>(S, 19, c);
-    auto& wrr = ref<R>(S, 20, c);
+    auto&& irr = ref<R>(S, 17, c);
+    auto&& vrr = ref<R>(S, 18, c);
+    auto&& urr = ref<R>(S, 19, c);
+    auto&& wrr = ref<R>(S, 20, c);
     const R tau_u = R(8);
     const R Fn = R(1e-12) * Vrel * irr - R(5 * 1e-13) / F * (R(0.5) * ical - R(0.2) * inaca);
     const R u_inf = R(1) / (R(1) + fexp(-(Fn - R(3.4175e-13)) / R(1.367e-15)));
@@ -835,9 +875,9 @@ __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
               ((R(1) + R(0.3) * fexp(-(u - R(7.9)) / R(5.0))) * (u - R(7.9)));
     }
     const R w_inf = R(1) - R(1) / (R(1) + fexp(-(u - R(40)) / R(17.0)));
-    const R urel = crn_gate(urr, u_inf, tau_u, dt);
-    const R vrel = crn_gate(vrr, v_inf, tau_v, dt);
-    const R wrel = crn_gate(wrr, w_inf, tau_w, dt);
+    const R urel = crn_gate<R>(urr, u_inf, tau_u, dt);
+    const R vrel = crn_gate<R>(vrr, v_inf, tau_v, dt);
+    const R wrel = crn_gate<R>(wrr, w_inf, tau_w, dt);
     urr = urel;
     vrr = vrel;
     wrr = wrel;

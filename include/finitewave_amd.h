@@ -300,8 +300,9 @@ int fw_batch_release(fw_batch* batch, int32_t slot);
 int fw_batch_block_nodes(const fw_batch* batch);
 /* Launch-shape variants of the batched step for a model (all give the same
  * results bit for bit): bit 0 = 4 consecutive nodes per work-item (models
- * 0-4), bit 1 = XCD-contiguous block map.  Writes up to `capacity` of them to
- * `variants` and returns how many there are. */
+ * 0-4), bit 1 = XCD-contiguous block map, bit 2 = state and new potential
+ * accessed with the non-temporal cache hint.  Writes up to `capacity` of them
+ * to `variants` and returns how many there are. */
 int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity);
 /* Selects the launch shape of fw_batch_run (rebuilds the block table when the
  * nodes per work-item change); returns FW_E_INVALID for a variant the model

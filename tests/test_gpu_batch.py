@@ -256,8 +256,10 @@ def test_autotuner_picks_a_variant_and_keeps_results(gpu, tmp_path, monkeypatch)
         b.admit(s)
     b.run_all()
     assert b.tuned and b.tuning is not None and set(b.tuning) == set(b.variants())
-    best = min(b.tuning, key=b.tuning.get)
-    assert b.variant in (best, max(b.tuning))  # the default shape wins near-ties
+    t = b.tuning
+    best = min(t, key=t.get)
+    # the fastest shape, or an XCD-contiguous one within 3 % of it
+    assert b.variant == best or (b.variant & 2 and t[b.variant] <= 1.03 * t[best])
     assert (tmp_path / "tune.json").exists()
     b.close()
     for s in sims:

@@ -353,8 +353,10 @@ def test_batch_launch_variants_per_model():
     import ctypes
     L = _native.lib()
     L.fw_batch_variants.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]
-    for model, expect in ((2, [0, 1, 2, 3]), (6, [0, 2]), (7, [0, 2])):
-        arr = (ctypes.c_int32 * 4)()
-        n = L.fw_batch_variants(model, arr, 4)
+    for model, expect in ((2, list(range(8))), (6, [0, 2, 4, 6]), (7, [0, 2, 4, 6])):
+        arr = (ctypes.c_int32 * 8)()
+        n = L.fw_batch_variants(model, arr, 8)
         assert list(arr[:n]) == expect
+    arr = (ctypes.c_int32 * 2)()
+    assert L.fw_batch_variants(2, arr, 2) == 8 and list(arr) == [0, 1]  # count beyond capacity
     assert L.fw_batch_variants(99, None, 0) == -1
