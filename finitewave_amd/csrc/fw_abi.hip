@@ -56,6 +56,11 @@ EcgLauncher ecg_launcher(int precision, int model) {
 
 __global__ void k_set_word(int32_t* p, int32_t v) { *p = v; }
 
+__global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) dst[q] = (float)src[q];
+}
+
 // Admission check of a batched tissue's codes: bit 0 of *bad when an updated
 // node (code & 3 == 1) lies on the outer layer, bit 1 when a node is frozen
 // (code bit 1).  Grid-stride over all n = ni * nj nodes.
@@ -257,6 +262,38 @@ int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
     delete p;
     return FW_E_NOMEM;
   }
+  // a kernel computing the diffusion in float reads float copies of the
+  // tables: 4-byte loads and no conversion per node
+  if (d.precision == FW_PREC_FP32 && !u_is_f64(d.model, d.precision)) {
+    bool synced = false;
+    for (int q = 0; q < need_fields; ++q) {
+      const fw_field& f = d.fields[q];
+      if (f.mode == FW_FIELD_UNIFORM) continue;
+      const int64_t len = f.mode == FW_FIELD_AXIS ? d.shape[f.axis] : N;
+      if (!synced) {
+        (void)hipDeviceSynchronize();  // the tables' uploads on any stream
+        synced = true;
+      }
+      e = hipMalloc(&p->ftab[q], len * sizeof(float));
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_to_float, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, 0,
+                           f.data, p->ftab[q], len);
+      if (e == hipSuccess) e = hipGetLastError();
+      if (e != hipSuccess) {
+        set_error("fw_plan_create: float field table: %s", hipGetErrorString(e));
+        fw_plan_destroy(p);
+        return FW_E_NOMEM;
+      }
+    }
+    if (synced) {
+      e = hipDeviceSynchronize();
+      if (e != hipSuccess) {
+        set_error("fw_plan_create: float field tables: %s", hipGetErrorString(e));
+        fw_plan_destroy(p);
+        return FW_E_HIP;
+      }
+    }
+  }
   *out = p;
   return FW_OK;
 }
@@ -277,6 +314,8 @@ int fw_plan_destroy(fw_plan* plan) {
   if (!plan) return FW_OK;
   if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
   if (plan->step_words) (void)hipFree(plan->step_words);
+  for (int q = 0; q < FW_MAX_FIELDS; ++q)
+    if (plan->ftab[q]) (void)hipFree(plan->ftab[q]);
   delete plan;
   return FW_OK;
 }

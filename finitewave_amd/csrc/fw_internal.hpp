@@ -22,6 +22,9 @@ struct fw_plan {
   void* graph_state[FW_MAX_STATE];
   int32_t graph_act_on;
   int32_t* graph_act;
+  // FP32 plans with a float potential: float copies of the tabulated
+  // half-step fields (made once by fw_plan_create)
+  float* ftab[FW_MAX_FIELDS];
 };
 
 namespace fw {

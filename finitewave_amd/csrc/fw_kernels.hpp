@@ -32,13 +32,54 @@ struct Field {
   int32_t axis;
   double value;
   const double* data;
+  float fvalue;        // the same in float, for kernels computing in float:
+  const float* fdata;  // the plan's float copy of data
 };
 
-__device__ __forceinline__ double fval(const Field& f, int64_t i, int64_t j,
-                                       int64_t k, int64_t c) {
-  if (f.mode == 0) return f.value;
-  if (f.mode == 1) return f.data[f.axis == 0 ? i : (f.axis == 1 ? j : k)];
-  return f.data[c];
+// a half-step field at node (i, j, k) = flat index c, in the type F the
+// diffusion is computed in; the mode is uniform across a launch
+template <typename F>
+__device__ __forceinline__ F fval(const Field& f, int64_t i, int64_t j, int64_t k, int64_t c) {
+  if (f.mode == 0) {
+    if constexpr (sizeof(F) == 4) return f.fvalue;
+    else return f.value;
+  }
+  const int64_t x = f.mode == 1 ? (f.axis == 0 ? i : (f.axis == 1 ? j : k)) : c;
+  if constexpr (sizeof(F) == 4) return ((const global_t<float>*)f.fdata)[x];
+  else return ((const global_t<double>*)f.data)[x];
+}
+
+// a field at node c and at c's lower neighbour along axis DIR (flat stride
+// `step`): one load when the field is uniform or tabulated along another axis
+template <typename F, int DIR>
+__device__ __forceinline__ void fval_pair(const Field& f, int64_t i, int64_t j, int64_t k,
+                                          int64_t c, int64_t step, F& at_c, F& below) {
+  at_c = fval<F>(f, i, j, k, c);
+  if (f.mode == 0 || (f.mode == 1 && f.axis != DIR))
+    below = at_c;
+  else
+    below = fval<F>(f, i - (DIR == 0), j - (DIR == 1), k - (DIR == 2), c - step);
+}
+
+// Division of 32-bit unsigned numerators by a divisor d >= 1 fixed for a
+// launch: q = (t + ((n - t) >> s1)) >> s2 with t = mulhi(n, m), Granlund and
+// Montgomery's round-up multiplier (Hacker's Delight, 2nd ed., 10-9), exact
+// for every n < 2^32.  Five integer operations instead of the emulated 64-bit
+// division; the host computes (m, s1, s2) once per launch.
+struct UDiv {
+  uint32_t m, s1, s2;
+};
+
+inline UDiv make_udiv(uint32_t d) {
+  uint32_t l = 0;
+  while ((uint64_t(1) << l) < d) ++l;
+  const uint64_t m = ((uint64_t(1) << 32) * ((uint64_t(1) << l) - d)) / d + 1;
+  return UDiv{(uint32_t)m, l < 1 ? l : 1u, l > 1 ? l - 1 : 0u};
+}
+
+__device__ __forceinline__ uint32_t udiv(uint32_t n, UDiv v) {
+  const uint32_t t = __umulhi(n, v.m);
+  return (t + ((n - t) >> v.s1)) >> v.s2;
 }
 
 // R: arithmetic type of the ionic update; U: storage type of u / u_new
@@ -69,7 +110,30 @@ struct KArgs {
   const int32_t* probe_var;
   double* probe_buf;
   int32_t probe_row;
+  int32_t idx32;      // N < 2^32: node coordinates by 32-bit division
+  UDiv div_jk, div_k;  // division by nj * nk and by nk
 };
+
+// (i, j, k) of flat node index c in a C-order grid (k = 0 in 2D)
+template <typename A>
+__device__ __forceinline__ void node_coords(const A& a, int64_t c, int64_t& i, int64_t& j,
+                                            int64_t& k) {
+  if (a.idx32) {
+    const uint32_t cc = (uint32_t)c;
+    const uint32_t ii = udiv(cc, a.div_jk);
+    const uint32_t rem = cc - ii * (uint32_t)(a.nj * a.nk);
+    const uint32_t jj = udiv(rem, a.div_k);
+    i = ii;
+    j = jj;
+    k = rem - jj * (uint32_t)a.nk;
+  } else {
+    const int64_t sjk = a.nj * a.nk;
+    i = c / sjk;
+    const int64_t rem = c - i * sjk;
+    j = rem / a.nk;
+    k = rem - j * a.nk;
+  }
+}
 
 // logical block id with each XCD owning a contiguous run of blocks
 __device__ __forceinline__ int64_t logical_block(uint32_t per_xcd) {
@@ -323,10 +387,9 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
     } else if constexpr (ST == 0) {
       const int mN = a.code[c - sN] & 1, mS = a.code[c + sN] & 1;
       const int mW = a.code[c - 1] & 1, mE = a.code[c + 1] & 1;
-      const double dN = fval(a.f[0], i - 1, j, 0, c - sN);
-      const double dS = fval(a.f[0], i, j, 0, c);
-      const double dW = fval(a.f[1], i, j - 1, 0, c - 1);
-      const double dE = fval(a.f[1], i, j, 0, c);
+      R dN, dS, dW, dE;
+      fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dS, dN);
+      fval_pair<R, 1>(a.f[1], i, j, 0, c, 1, dE, dW);
       if constexpr (EXACT) {
         double W[5];
         W[0] = iso_comp(dN, mN, mS);
@@ -356,14 +419,11 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
           for (int dk = -1; dk <= 1; ++dk) M.m[(di + 1) * 9 + (dj + 1) * 3 + dk + 1] = 0;
           M.m[(di + 1) * 9 + (dj + 1) * 3 + 1] = a.code[c + di * sN + dj] & 1;
         }
-      const double dxx0 = fval(a.f[0], i - 1, j, 0, c - sN);
-      const double dxx1 = fval(a.f[0], i, j, 0, c);
-      const double dxy0 = fval(a.f[1], i - 1, j, 0, c - sN);
-      const double dxy1 = fval(a.f[1], i, j, 0, c);
-      const double dyx0 = fval(a.f[2], i, j - 1, 0, c - 1);
-      const double dyx1 = fval(a.f[2], i, j, 0, c);
-      const double dyy0 = fval(a.f[3], i, j - 1, 0, c - 1);
-      const double dyy1 = fval(a.f[3], i, j, 0, c);
+      R dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1;
+      fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dxx1, dxx0);
+      fval_pair<R, 0>(a.f[1], i, j, 0, c, sN, dxy1, dxy0);
+      fval_pair<R, 1>(a.f[2], i, j, 0, c, 1, dyx1, dyx0);
+      fval_pair<R, 1>(a.f[3], i, j, 0, c, 1, dyy1, dyy0);
       if constexpr (EXACT) {
         double W[9];
         wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
@@ -417,12 +477,10 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
       const int mXm = a.code[c - si] & 1, mXp = a.code[c + si] & 1;
       const int mYm = a.code[c - sj] & 1, mYp = a.code[c + sj] & 1;
       const int mZm = a.code[c - 1] & 1, mZp = a.code[c + 1] & 1;
-      const double dXm = fval(a.f[0], i - 1, j, k, c - si);
-      const double dXp = fval(a.f[0], i, j, k, c);
-      const double dYm = fval(a.f[1], i, j - 1, k, c - sj);
-      const double dYp = fval(a.f[1], i, j, k, c);
-      const double dZm = fval(a.f[2], i, j, k - 1, c - 1);
-      const double dZp = fval(a.f[2], i, j, k, c);
+      R dXm, dXp, dYm, dYp, dZm, dZp;
+      fval_pair<R, 0>(a.f[0], i, j, k, c, si, dXp, dXm);
+      fval_pair<R, 1>(a.f[1], i, j, k, c, sj, dYp, dYm);
+      fval_pair<R, 2>(a.f[2], i, j, k, c, 1, dZp, dZm);
       if constexpr (EXACT) {
         double W[7];
         W[0] = iso_comp(dXm, mXm, mXp);
@@ -460,25 +518,16 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
                 corner ? 0 : (a.code[c + di * si + dj * sj + dk] & 1);
           }
       // fields: 0 xx 1 xy 2 xz 3 yx 4 yy 5 yz 6 zx 7 zy 8 zz
-      double d[6][3];
-      d[0][0] = fval(a.f[0], i - 1, j, k, c - si);
-      d[0][1] = fval(a.f[1], i - 1, j, k, c - si);
-      d[0][2] = fval(a.f[2], i - 1, j, k, c - si);
-      d[1][0] = fval(a.f[0], i, j, k, c);
-      d[1][1] = fval(a.f[1], i, j, k, c);
-      d[1][2] = fval(a.f[2], i, j, k, c);
-      d[2][0] = fval(a.f[4], i, j - 1, k, c - sj);
-      d[2][1] = fval(a.f[3], i, j - 1, k, c - sj);
-      d[2][2] = fval(a.f[5], i, j - 1, k, c - sj);
-      d[3][0] = fval(a.f[4], i, j, k, c);
-      d[3][1] = fval(a.f[3], i, j, k, c);
-      d[3][2] = fval(a.f[5], i, j, k, c);
-      d[4][0] = fval(a.f[8], i, j, k - 1, c - 1);
-      d[4][1] = fval(a.f[6], i, j, k - 1, c - 1);
-      d[4][2] = fval(a.f[7], i, j, k - 1, c - 1);
-      d[5][0] = fval(a.f[8], i, j, k, c);
-      d[5][1] = fval(a.f[6], i, j, k, c);
-      d[5][2] = fval(a.f[7], i, j, k, c);
+      R d[6][3];
+      fval_pair<R, 0>(a.f[0], i, j, k, c, si, d[1][0], d[0][0]);
+      fval_pair<R, 0>(a.f[1], i, j, k, c, si, d[1][1], d[0][1]);
+      fval_pair<R, 0>(a.f[2], i, j, k, c, si, d[1][2], d[0][2]);
+      fval_pair<R, 1>(a.f[4], i, j, k, c, sj, d[3][0], d[2][0]);
+      fval_pair<R, 1>(a.f[3], i, j, k, c, sj, d[3][1], d[2][1]);
+      fval_pair<R, 1>(a.f[5], i, j, k, c, sj, d[3][2], d[2][2]);
+      fval_pair<R, 2>(a.f[8], i, j, k, c, 1, d[5][0], d[4][0]);
+      fval_pair<R, 2>(a.f[6], i, j, k, c, 1, d[5][1], d[4][1]);
+      fval_pair<R, 2>(a.f[7], i, j, k, c, 1, d[5][2], d[4][2]);
       if constexpr (EXACT) {
         double W[19];
         wts_aniso3d(W, M, d);
@@ -577,11 +626,8 @@ __global__ __launch_bounds__(kBlock) void k_step(const KArgs<R, UType<MODEL, R>>
   if ((code & 3) == 1) {
     // node coordinates; skipped work for nodes on the outer layer keeps every
     // neighbour access inside the grid whatever the codes say
-    const int64_t sjk = a.nj * a.nk;
-    const int64_t i = c / sjk;
-    const int64_t rem = c - i * sjk;
-    const int64_t j = rem / a.nk;
-    const int64_t k = rem - j * a.nk;
+    int64_t i, j, k;
+    node_coords(a, c, i, j, k);
     bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
     if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
     if (interior) {
@@ -630,8 +676,8 @@ __global__ __launch_bounds__(kBlock) void k_ecg(const KArgs<R, UType<MODEL, R>> 
 #pragma unroll
   for (int e = 0; e < kEcgPoints; ++e) acc[e] = 0.0;
   if (c < a.N && (a.code[c] & 3) == 1) {
-    const int64_t sjk = a.nj * a.nk;
-    const int64_t i = c / sjk, rem = c - i * sjk, j = rem / a.nk, k = rem - j * a.nk;
+    int64_t i, j, k;
+    node_coords(a, c, i, j, k);
     bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
     if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
     if (interior) {

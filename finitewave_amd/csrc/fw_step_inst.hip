@@ -40,6 +40,9 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
   a.nj = d.shape[1];
   a.nk = d.ndim == 3 ? d.shape[2] : 1;
   a.N = plan->N;
+  a.idx32 = a.N <= 0xffffffffll;
+  a.div_jk = make_udiv(a.idx32 ? (uint32_t)(a.nj * a.nk) : 1u);
+  a.div_k = make_udiv(a.idx32 ? (uint32_t)a.nk : 1u);
   a.nblocks = plan->nblocks;
   a.per_xcd = plan->per_xcd;
   for (int q = 0; q < 9; ++q) {
@@ -48,8 +51,10 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
       a.f[q].axis = d.fields[q].axis;
       a.f[q].value = d.fields[q].value;
       a.f[q].data = d.fields[q].data;
+      a.f[q].fvalue = (float)d.fields[q].value;
+      a.f[q].fdata = plan->ftab[q];
     } else {
-      a.f[q] = Field{0, 0, 0.0, nullptr};
+      a.f[q] = Field{0, 0, 0.0, nullptr, 0.0f, nullptr};
     }
   }
   if (d.stencil == FW_STENCIL_WEIGHTS) a.f[0].mode = plan->K;

@@ -4,8 +4,10 @@
  * Plain C: pointers, sizes, status codes.  Every array argument is a device
  * pointer (HIP), C-order over the grid shape, unless stated otherwise.  No
  * entry point allocates device memory except fw_plan_create (a small
- * parameter block), fw_batch_create (the tissue descriptor table) and
- * fw_batch_set_active (the block table, when it grows).  `stream` is a
+ * parameter block and, for an FP32 plan whose potential is float, float
+ * copies of its tabulated half-step fields, read once at creation),
+ * fw_batch_create (the tissue descriptor table) and fw_batch_set_active (the
+ * block table, when it grows).  `stream` is a
  * hipStream_t passed as void* (NULL = the default stream).  All launches are
  * asynchronous on that stream.
  *
