@@ -313,6 +313,7 @@ int fw_plan_set_weights_k(fw_plan* plan, int32_t K) {
 int fw_plan_destroy(fw_plan* plan) {
   if (!plan) return FW_OK;
   if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
+  if (plan->graph_long) (void)hipGraphExecDestroy(plan->graph_long);
   if (plan->step_words) (void)hipFree(plan->step_words);
   for (int q = 0; q < FW_MAX_FIELDS; ++q)
     if (plan->ftab[q]) (void)hipFree(plan->ftab[q]);
@@ -372,6 +373,54 @@ int fw_run(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step
   return FW_OK;
 }
 
+// Pairs of steps per launch of the long graph: a graph launch costs several
+// microseconds on the host, as much as a whole step of a small grid, so long
+// runs replay 64 steps per launch.
+constexpr int32_t kGraphLongPairs = 32;
+
+// Captures `pairs` pairs of steps (u -> u_new -> u) into an executable graph.
+// Each kernel reads its step from one device word and writes step + 1 to the
+// other, so a replay continues from whatever step the words hold.
+static int capture_pairs(fw_plan* plan, void* u, void* u_new, const StatePtrs& st,
+                         int32_t* act, int32_t act_on, double act_thr, int32_t pairs,
+                         hipGraphExec_t* out) {
+  StepLauncher L = step_launcher(plan->desc.precision, plan->desc.model);
+  hipStream_t cap;
+  if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+    set_error("fw_run_graph: stream creation failed");
+    return FW_E_HIP;
+  }
+  int rc = FW_OK;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    TrackArgs tr{};
+    tr.act = act;
+    tr.act_thr = act_thr;
+    tr.act_on = act_on;
+    tr.probe_row = -1;
+    for (int32_t q = 0; q < pairs && !rc; ++q) {
+      tr.step_in = plan->step_words;
+      tr.step_out = plan->step_words + 1;
+      rc = L(plan, u, u_new, st, tr, cap);
+      tr.step_in = plan->step_words + 1;
+      tr.step_out = plan->step_words;
+      if (!rc) rc = L(plan, u_new, u, st, tr, cap);
+    }
+    e = hipStreamEndCapture(cap, &g);
+  }
+  hipGraphExec_t ex = nullptr;
+  if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess || rc || !ex) {
+    set_error("fw_run_graph: capture failed (%s)", hipGetErrorString(e));
+    return rc ? rc : FW_E_HIP;
+  }
+  *out = ex;
+  return FW_OK;
+}
+
 int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
                  int32_t n_steps, const fw_track_desc* track, void* stream) {
   int rc = check_run_args(plan, u, u_new, state, n_steps, track);
@@ -392,59 +441,44 @@ int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_
           return FW_E_UNSUPPORTED;
         }
   }
+  const double act_thr = track ? track->act_thr : 0.0;
   const int ns = model_n_state(plan->desc.model);
   const int32_t pairs = n_steps / 2;
   hipStream_t s = as_stream(stream);
-  // cached graph still valid for these buffers?
+  // cached graphs still valid for these buffers?
   bool reuse = plan->graph && plan->graph_u == u && plan->graph_un == u_new &&
                plan->graph_act == act && plan->graph_act_on == act_on;
   for (int k = 0; reuse && k < ns; ++k) reuse = plan->graph_state[k] == state[k];
+  const StatePtrs st = make_state(state, ns);
   if (!reuse && pairs > 0) {
-    if (plan->graph) {
-      (void)hipGraphExecDestroy(plan->graph);
-      plan->graph = nullptr;
-    }
-    StepLauncher L = step_launcher(plan->desc.precision, plan->desc.model);
-    const StatePtrs st = make_state(state, ns);
-    hipStream_t cap;
-    if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
-      set_error("fw_run_graph: stream creation failed");
-      return FW_E_HIP;
-    }
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-      TrackArgs tr{};
-      tr.act = act;
-      tr.act_thr = track ? track->act_thr : 0.0;
-      tr.act_on = act_on;
-      tr.probe_row = -1;
-      tr.step_in = plan->step_words;
-      tr.step_out = plan->step_words + 1;
-      rc = L(plan, u, u_new, st, tr, cap);
-      tr.step_in = plan->step_words + 1;
-      tr.step_out = plan->step_words;
-      if (!rc) rc = L(plan, u_new, u, st, tr, cap);
-      e = hipStreamEndCapture(cap, &g);
-    }
-    hipGraphExec_t ex = nullptr;
-    if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    if (g) (void)hipGraphDestroy(g);
-    (void)hipStreamDestroy(cap);
-    if (e != hipSuccess || rc || !ex) {
-      set_error("fw_run_graph: capture failed (%s)", hipGetErrorString(e));
-      return rc ? rc : FW_E_HIP;
-    }
-    plan->graph = ex;
+    if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
+    if (plan->graph_long) (void)hipGraphExecDestroy(plan->graph_long);
+    plan->graph = nullptr;
+    plan->graph_long = nullptr;
+    rc = capture_pairs(plan, u, u_new, st, act, act_on, act_thr, 1, &plan->graph);
+    if (rc) return rc;
     plan->graph_u = u;
     plan->graph_un = u_new;
     plan->graph_act = act;
     plan->graph_act_on = act_on;
     for (int k = 0; k < FW_MAX_STATE; ++k) plan->graph_state[k] = k < ns ? state[k] : nullptr;
   }
+  if (pairs >= kGraphLongPairs && !plan->graph_long) {
+    rc = capture_pairs(plan, u, u_new, st, act, act_on, act_thr, kGraphLongPairs,
+                       &plan->graph_long);
+    if (rc) return rc;
+  }
   if (pairs > 0) {
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, plan->step_words, (int32_t)step0);
-    for (int32_t q = 0; q < pairs; ++q) {
+    int32_t q = 0;
+    for (; q + kGraphLongPairs <= pairs; q += kGraphLongPairs) {
+      hipError_t e = hipGraphLaunch(plan->graph_long, s);
+      if (e != hipSuccess) {
+        set_error("fw_run_graph: launch failed: %s", hipGetErrorString(e));
+        return FW_E_HIP;
+      }
+    }
+    for (; q < pairs; ++q) {
       hipError_t e = hipGraphLaunch(plan->graph, s);
       if (e != hipSuccess) {
         set_error("fw_run_graph: launch failed: %s", hipGetErrorString(e));

@@ -15,7 +15,8 @@ struct fw_plan {
   uint32_t grid;     // launched blocks = per_xcd * 8
   int32_t K;         // weights per node for FW_STENCIL_WEIGHTS
   int32_t* step_words;  // device int32[2] for graph replay
-  hipGraphExec_t graph;
+  hipGraphExec_t graph;       // two steps (u -> u_new -> u)
+  hipGraphExec_t graph_long;  // kGraphLongPairs such pairs, for long runs
   int32_t graph_steps;
   const void* graph_u;
   const void* graph_un;
