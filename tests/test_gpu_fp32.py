@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import finitewave_amd as fw
+from tests.scenarios import layered_fibers, random_fibers
 
 
 def _apd90(u, dt, threshold_up):
@@ -124,6 +125,74 @@ def test_fp32_one_step_from_random_state_matches_fp64_oracle(gpu, seed):
     f[..., 0], f[..., 1] = np.cos(ang), np.sin(ang)
     model, sim = make_pair("barkley", 2, shape, stencil="aniso", fibrosis=0.3, fibers=f,
                            precision="fp32")
+    u0 = rng.uniform(-1, 1, size=shape)
+    u0[model.cardiac_tissue.mesh != 1] = 0.0
+    model.u[...] = u0
+    sim.u[...] = u0
+    eng = Engine(model)
+    eng.upload()
+    eng.run_steps(0, 1, use_graph=False)
+    u_gpu = eng.u_dev().cpu().numpy().astype(np.float64)
+    sim.one_step()
+    sim.u, sim.u_new = sim.u_new, sim.u
+    act = model.cardiac_tissue.mesh == 1
+    assert float(np.max(np.abs(u_gpu[act] - sim.u[act]))) <= 1e-5
+
+
+def _layer_profile(n, lo, hi):
+    return lo + (hi - lo) * np.linspace(0.0, 1.0, n)
+
+
+FIELD_CASES = {
+    # name: (dim, shape, stencil, builder of (fibers, conductivity), expected field modes)
+    "3d-aniso-fibres-along-k": (3, (12, 13, 14), "aniso",
+                                lambda s: (layered_fibers(s), 1.0), {1}),
+    "3d-aniso-fibres-per-node": (3, (11, 12, 13), "aniso",
+                                 lambda s: (random_fibers(s, seed=4), 1.0), {2}),
+    "3d-iso-conductivity-along-k": (3, (12, 11, 15), "iso",
+                                    lambda s: (None, np.broadcast_to(
+                                        _layer_profile(s[2], 0.4, 1.0), s).copy()), {1}),
+    "2d-aniso-fibres-along-i": (2, (31, 29), "aniso",
+                                lambda s: (_fibres_2d(np.linspace(0, np.pi, s[0])[:, None],
+                                                      s), 1.0), {0, 1}),
+    "2d-aniso-fibres-along-j": (2, (27, 33), "aniso",
+                                lambda s: (_fibres_2d(np.linspace(0, np.pi, s[1])[None, :],
+                                                      s), 1.0), {0, 1}),
+    "2d-iso-conductivity-along-j": (2, (25, 30), "iso",
+                                    lambda s: (None, np.broadcast_to(
+                                        _layer_profile(s[1], 0.3, 1.0)[None, :], s).copy()),
+                                    {0, 1}),
+}
+
+
+def _fibres_2d(ang, shape):
+    f = np.zeros((*shape, 2))
+    f[..., 0] = np.cos(ang)
+    f[..., 1] = np.sin(ang)
+    return f
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", sorted(FIELD_CASES))
+def test_fp32_one_step_with_tabulated_fields_matches_fp64_oracle(gpu, case):
+    """One FP32 fused step with half-step fields stored as a table along one
+    axis or per node (the plan's float copies, and one load where a table's
+    axis differs from the face direction), 2D and 3D, iso and aniso, from
+    random potentials on a fibrotic mesh, against the oracle's FP64 step.
+    Tolerance 1e-5 absolute, as for uniform fields; a field read at the wrong
+    node or face is off by 1e-3 or more."""
+    from finitewave_amd.engine import Engine
+    from finitewave_amd.stencil import compress_field
+    from tests.scenarios import make_pair
+    dim, shape, stencil, build, modes = FIELD_CASES[case]
+    fibers, cond = build(shape)
+    model, sim = make_pair("barkley", dim, shape, stencil=stencil, fibrosis=0.2, fibers=fibers,
+                           conductivity=cond, precision="fp32")
+    # the case exercises the field storage it names (uniform = 0, table = 1, full = 2)
+    got = {compress_field(a)[0] for a in model.stencil.half_step_fields(model,
+                                                                        model.cardiac_tissue)}
+    assert got & {1, 2} and got <= modes | {0}, got
+    rng = np.random.RandomState(len(case))
     u0 = rng.uniform(-1, 1, size=shape)
     u0[model.cardiac_tissue.mesh != 1] = 0.0
     model.u[...] = u0
