@@ -107,9 +107,11 @@ FW_HD inline void two_sum(double a, double b, double& s, double& e) {
 }
 
 FW_HD inline double dd_log(double x) {
-  if (x != x || x < 0.0) return NAN;
-  if (x == 0.0) return -HUGE_VAL;
-  if (x == HUGE_VAL) return x;
+  if (!(x > 0.0 && x < HUGE_VAL)) {  // zero, negative, infinite, NaN
+    if (x == 0.0) return -HUGE_VAL;
+    if (x == HUGE_VAL) return x;
+    return NAN;
+  }
   int ex;
   double y = frexp(x, &ex);  // x = y * 2^ex, y in [0.5, 1)
   y *= 2.0;
@@ -121,14 +123,21 @@ FW_HD inline double dd_log(double x) {
   const int j = (int)rint(y * 128.0);  // 90..181
   const double F = (double)j * (1.0 / 128.0);
   const double f = y - F;  // exact (Sterbenz)
-  const double u = f / F;
-  const double ulo = fma(-u, F, f) / F;
-  // log1p(u) - u
-  const double q =
-      u * u *
-      (-0.5 +
-       u * (1.0 / 3 +
-            u * (-0.25 + u * (0.2 + u * (-1.0 / 6 + u * (1.0 / 7 + u * (-0.125 + u * (1.0 / 9))))))));
+  // u + ulo = f / F to about 2^-105 relative: u = f RN(128 / j) is within
+  // 1.5 ulp of f / F, and the remainder f - u F (one fma) times RN(128 / j)
+  // supplies the rest
+  const double inv = mt::kLogInvF[j - 64];
+  const double u = f * inv;
+  const double ulo = fma(-u, F, f) * inv;
+  // log1p(u) - u, |u| <= 2^-7.5: its rounding enters at 2^-68 of the result
+  const double p = fma(
+      u,
+      fma(u,
+          fma(u, fma(u, fma(u, fma(u, fma(u, 1.0 / 9, -0.125), 1.0 / 7), -1.0 / 6), 0.2),
+              -0.25),
+          1.0 / 3),
+      -0.5);
+  const double q = u * u * p;
   const double mh = (double)m * mt::kLn2Hi;  // exact
   const double ml = (double)m * mt::kLn2Lo;
   double s1, e1, s2, e2;
