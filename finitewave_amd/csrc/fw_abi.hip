@@ -54,6 +54,26 @@ EcgLauncher ecg_launcher(int precision, int model) {
   return table[precision][model];
 }
 
+void derive_params(int model, Params& P) {
+  if (model != 6) return;
+  // TP06 parameter order (tp06_2d.py:1073 call)
+  const double ko = P.v[0], cao = P.v[1], nao = P.v[2], Vc = P.v[3], Vsr = P.v[4],
+               Vss = P.v[5], R = P.v[24], F = P.v[25], T = P.v[26], KmK = P.v[34],
+               KmNai = P.v[40], KmCa = P.v[41];
+  double* d = P.v + kTp06Derived;
+  d[kInvVcF] = 1.0 / (Vc * F);
+  d[kInvVssF2] = 1.0 / (2 * Vss * F);
+  d[kFFoverRT] = F * F / (R * T);
+  d[kRT] = R * T;
+  d[kInvRT] = 1.0 / (R * T);
+  d[kInvNaCaK] = 1.0 / (KmNai * KmNai * KmNai + nao * nao * nao);
+  d[kInvKmCaCao] = 1.0 / (KmCa + cao);
+  d[kKoOverKoKmK] = ko / (ko + KmK);
+  d[kSqrtKo54] = sqrt(ko / 5.4);
+  d[kVcOverVss] = Vc / Vss;
+  d[kVsrOverVss] = Vsr / Vss;
+}
+
 __global__ void k_set_word(int32_t* p, int32_t v) { *p = v; }
 
 __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
@@ -168,6 +188,7 @@ int fw_ionic(int32_t model, int32_t precision, void* u_new, const void* u, void*
     }
   Params P;
   for (int q = 0; q < FW_MAX_PARAMS; ++q) P.v[q] = q < n_params ? params[q] : 0.0;
+  derive_params(model, P);
   return launch_ionic(model, precision, u_new, u, make_state(state, n_state), indexes,
                       n_indexes, dt, P, as_stream(stream));
 }
@@ -627,6 +648,7 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   const double mant = frexp(dr2, &e2);
   b->inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
   for (int q = 0; q < FW_MAX_PARAMS; ++q) b->P.v[q] = q < n_params ? params[q] : 0.0;
+  derive_params(model, b->P);
   b->host = new (std::nothrow) fw_tissue_desc[max_tissues]();
   if (!b->host || !b->active ||
       hipMalloc(&b->table, sizeof(BatchTissue) * max_tissues) != hipSuccess ||

@@ -67,16 +67,39 @@ __device__ __forceinline__ float exprel(float y) {
 // a/b; the remainder a - b q1 is exact).  Holds while a/b is a normal number
 // or zero (a zero quotient comes out +0).  FP32 path: the plain division.
 template <typename R>
-__device__ __forceinline__ R divc(R a, R b) {
-  if constexpr (sizeof(R) == 8) {
-    const R y = R(1) / b;
-    const R q0 = a * y;
-    const R q1 = fma(fma(-b, q0, a), y, q0);
-    return fma(fma(-b, q1, a), y, q1);
-  } else {
-    return a / b;
-  }
+__device__ __forceinline__ R div_rn(R a, R b, R y) {  // y = 1/b correctly rounded
+  const R q0 = a * y;
+  const R q1 = fma(fma(-b, q0, a), y, q0);
+  return fma(fma(-b, q1, a), y, q1);
 }
+
+template <typename R>
+__device__ __forceinline__ R divc(R a, R b) {
+  if constexpr (sizeof(R) == 8) return div_rn(a, b, R(1) / b);
+  else return a / b;
+}
+
+// TP06 quantities that are the same for every node (functions of the
+// parameters only), appended after its 49 parameters by derive_params on the
+// host, in the operation order of the expressions they replace: IEEE +, *, /
+// and sqrt round the same on host and device, so the FP64 path's values are
+// unchanged while each node saves ten divisions and a square root.
+constexpr int kTp06Derived = 49;
+enum Tp06Derived {
+  kInvVcF,      // 1.0 / (Vc * F)
+  kInvVssF2,    // 1.0 / (2 * Vss * F)
+  kFFoverRT,    // F * F / (R * T)
+  kRT,          // R * T
+  kInvRT,       // 1.0 / (R * T)
+  kInvNaCaK,    // 1.0 / (KmNai * KmNai * KmNai + nao * nao * nao)
+  kInvKmCaCao,  // 1.0 / (KmCa + cao)
+  kKoOverKoKmK, // ko / (ko + KmK)
+  kSqrtKo54,    // sqrt(ko / 5.4)
+  kVcOverVss,   // Vc / Vss
+  kVsrOverVss,  // Vsr / Vss
+  kTp06NDerived
+};
+static_assert(kTp06Derived + kTp06NDerived <= 64, "parameter block size");
 
 // state arrays live in global memory: accesses through the global address
 // space compile to global (not flat) loads and stores
@@ -414,8 +437,15 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
           KmCa = (R)P.v[41], ksat = (R)P.v[42], n_ = (R)P.v[43],
           gpca = (R)P.v[44], KpCa = (R)P.v[45], gpk = (R)P.v[46],
           gto = (R)P.v[47], gks = (R)P.v[48];
-  const D inverseVcF = 1.0 / (Vc * Fd);
-  const D inversevssF2 = 1.0 / (2 * Vss * Fd);
+  constexpr bool F64 = sizeof(R) == 8;
+  const double* Dv = P.v + kTp06Derived;  // uniform quantities (FP64 path)
+  const D inverseVcF = F64 ? Dv[kInvVcF] : 1.0 / (Vc * Fd);
+  const D inversevssF2 = F64 ? Dv[kInvVssF2] : 1.0 / (2 * Vss * Fd);
+  // x / (R T) for a node-dependent x
+  auto overRT = [&](R x) -> R {
+    if constexpr (F64) return div_rn(x, (R)Dv[kRT], (R)Dv[kInvRT]);
+    else return x / (R_ * T);
+  };
 
   auto&& cair = ref<D>(S, 0, c);
   auto&& casrr = ref<D>(S, 1, c);
@@ -516,9 +546,9 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
       ical = gcal * d * f * f2 * fcass * R(4) * (F * F / (R_ * T)) *
              (R(0.25) * fexp(X) * cass - cao) / (R(2) * F / (R_ * T) * exprel(X));
     } else {
-      ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * (F * F / (R_ * T)) *
-             (R(0.25) * fexp(R(2) * (u - R(15)) * F / (R_ * T)) * cass - cao) /
-             (fexp(R(2) * (u - R(15)) * F / (R_ * T)) - R(1.0));
+      const R e = fexp(overRT(R(2) * (u - R(15)) * F));
+      ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * (R)Dv[kFFoverRT] *
+             (R(0.25) * e * cass - cao) / (e - R(1.0));
     }
   }
   // calc_ito
@@ -554,7 +584,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     const R xr2 = xr2_inf - (xr2_inf - x2r) * fexp(-dt / tau_xr2);
     x1r = xr1;
     x2r = xr2;
-    ikr = gkr * fsqrt(ko / R(5.4)) * xr1 * xr2 * (u - Ek);
+    ikr = gkr * (F64 ? (R)Dv[kSqrtKo54] : fsqrt(ko / R(5.4))) * xr1 * xr2 * (u - Ek);
   }
   // calc_iks
   R iks;
@@ -576,16 +606,17 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   const R rec_iK1 = ak1 / (ak1 + bk1);
   const R ik1 = gk1 * rec_iK1 * (u - Ek);
   // calc_inaca
+  const R e_n1 = fexp(overRT((n_ - R(1)) * u * F));
   const R inaca =
-      knaca * (R(1.0) / (KmNai * KmNai * KmNai + nao * nao * nao)) *
-      (R(1.0) / (KmCa + cao)) *
-      (R(1.0) / (R(1) + ksat * fexp((n_ - R(1)) * u * F / (R_ * T)))) *
-      (fexp(n_ * u * F / (R_ * T)) * nai * nai * nai * cao -
-       fexp((n_ - R(1)) * u * F / (R_ * T)) * nao * nao * nao * cai * R(2.5));
+      knaca *
+      (F64 ? (R)Dv[kInvNaCaK] : R(1.0) / (KmNai * KmNai * KmNai + nao * nao * nao)) *
+      (F64 ? (R)Dv[kInvKmCaCao] : R(1.0) / (KmCa + cao)) * (R(1.0) / (R(1) + ksat * e_n1)) *
+      (fexp(overRT(n_ * u * F)) * nai * nai * nai * cao - e_n1 * nao * nao * nao * cai * R(2.5));
   // calc_inak
-  const R rec_iNaK = R(1.0) / (R(1.0) + R(0.1245) * fexp(R(-0.1) * u * F / (R_ * T)) +
-                               R(0.0353) * fexp(-u * F / (R_ * T)));
-  const R inak = knak * (ko / (ko + KmK)) * (nai / (nai + KmNa)) * rec_iNaK;
+  const R rec_iNaK = R(1.0) / (R(1.0) + R(0.1245) * fexp(overRT(R(-0.1) * u * F)) +
+                               R(0.0353) * fexp(overRT(-u * F)));
+  const R inak =
+      knak * (F64 ? (R)Dv[kKoOverKoKmK] : ko / (ko + KmK)) * (nai / (nai + KmNa)) * rec_iNaK;
   const R ipca = gpca * cai / (KpCa + cai);
   const R rec_ipK = R(1.0) / (R(1.0) + fexp(divc(R(25) - u, R(5.98))));
   const R ipk = gpk * rec_ipK * (u - Ek);
@@ -622,7 +653,8 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_cass
   {
     const D CaSSBuf = Bufss * cassD / (cassD + Kbufss);
-    const D dCaSS = dtD * (-ixfer * (Vc / Vss) + irel * (Vsr / Vss) +
+    const D dCaSS = dtD * (-ixfer * (F64 ? Dv[kVcOverVss] : Vc / Vss) +
+                           irel * (F64 ? Dv[kVsrOverVss] : Vsr / Vss) +
                            -(D)ical * inversevssF2 * CAPACITANCE);
     const D bcss = Bufss - CaSSBuf - dCaSS - cassD + Kbufss;
     const D ccss = Kbufss * (CaSSBuf + dCaSS + cassD);
