@@ -610,8 +610,19 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
   }
 }
 
+// Waves per SIMD the compiler must leave room for.  The 2D FP64 TP06 step
+// would take 136-156 registers (3 waves); a 128-register budget fits 4 at the
+// cost of 6 spilled registers and is 2.6 % faster; 96 registers (5 waves)
+// spill 36 and are 9 % slower.  The 3D anisotropic one would spill 74 under
+// the same budget.  Other steps keep the compiler's choice.
+template <int NDIM, int MODEL, int PREC>
+constexpr int step_min_waves() {
+  return (NDIM == 2 && MODEL == 6 && PREC == 0) ? 4 : 1;
+}
+
 template <int NDIM, int ST, int MODEL, typename R, int PREC>
-__global__ __launch_bounds__(kBlock) void k_step(const KArgs<R, UType<MODEL, R>> a) {
+__global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void k_step(
+    const KArgs<R, UType<MODEL, R>> a) {
   using U = UType<MODEL, R>;
   const int64_t lb = logical_block(a.per_xcd);
   if (lb >= a.nblocks) return;
