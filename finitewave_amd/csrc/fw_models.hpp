@@ -83,7 +83,8 @@ __device__ __forceinline__ R divc(R a, R b) {
 // parameters only), appended after its 49 parameters by derive_params on the
 // host, in the operation order of the expressions they replace: IEEE +, *, /
 // and sqrt round the same on host and device, so the FP64 path's values are
-// unchanged while each node saves ten divisions and a square root.
+// unchanged while each node saves ten divisions and a square root.  The FP32
+// path rounds the same double values to float once.
 constexpr int kTp06Derived = 49;
 enum Tp06Derived {
   kInvVcF,      // 1.0 / (Vc * F)
@@ -438,9 +439,9 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
           gpca = (R)P.v[44], KpCa = (R)P.v[45], gpk = (R)P.v[46],
           gto = (R)P.v[47], gks = (R)P.v[48];
   constexpr bool F64 = sizeof(R) == 8;
-  const double* Dv = P.v + kTp06Derived;  // uniform quantities (FP64 path)
-  const D inverseVcF = F64 ? Dv[kInvVcF] : 1.0 / (Vc * Fd);
-  const D inversevssF2 = F64 ? Dv[kInvVssF2] : 1.0 / (2 * Vss * Fd);
+  const double* Dv = P.v + kTp06Derived;  // uniform quantities (derive_params)
+  const D inverseVcF = Dv[kInvVcF];
+  const D inversevssF2 = Dv[kInvVssF2];
   // x / (R T) for a node-dependent x
   auto overRT = [&](R x) -> R {
     if constexpr (F64) return div_rn(x, (R)Dv[kRT], (R)Dv[kInvRT]);
@@ -543,7 +544,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     if constexpr (sizeof(R) == 4) {
       // (u - 15) / (e^X - 1) = 1 / ((2F/RT) * exprel(X)),  X = 2(u - 15)F/RT
       const R X = R(2) * (u - R(15)) * F / (R_ * T);
-      ical = gcal * d * f * f2 * fcass * R(4) * (F * F / (R_ * T)) *
+      ical = gcal * d * f * f2 * fcass * R(4) * (R)Dv[kFFoverRT] *
              (R(0.25) * fexp(X) * cass - cao) / (R(2) * F / (R_ * T) * exprel(X));
     } else {
       const R e = fexp(overRT(R(2) * (u - R(15)) * F));
@@ -584,7 +585,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     const R xr2 = xr2_inf - (xr2_inf - x2r) * fexp(-dt / tau_xr2);
     x1r = xr1;
     x2r = xr2;
-    ikr = gkr * (F64 ? (R)Dv[kSqrtKo54] : fsqrt(ko / R(5.4))) * xr1 * xr2 * (u - Ek);
+    ikr = gkr * (R)Dv[kSqrtKo54] * xr1 * xr2 * (u - Ek);
   }
   // calc_iks
   R iks;
@@ -609,14 +610,13 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   const R e_n1 = fexp(overRT((n_ - R(1)) * u * F));
   const R inaca =
       knaca *
-      (F64 ? (R)Dv[kInvNaCaK] : R(1.0) / (KmNai * KmNai * KmNai + nao * nao * nao)) *
-      (F64 ? (R)Dv[kInvKmCaCao] : R(1.0) / (KmCa + cao)) * (R(1.0) / (R(1) + ksat * e_n1)) *
+      (R)Dv[kInvNaCaK] * (R)Dv[kInvKmCaCao] * (R(1.0) / (R(1) + ksat * e_n1)) *
       (fexp(overRT(n_ * u * F)) * nai * nai * nai * cao - e_n1 * nao * nao * nao * cai * R(2.5));
   // calc_inak
   const R rec_iNaK = R(1.0) / (R(1.0) + R(0.1245) * fexp(overRT(R(-0.1) * u * F)) +
                                R(0.0353) * fexp(overRT(-u * F)));
   const R inak =
-      knak * (F64 ? (R)Dv[kKoOverKoKmK] : ko / (ko + KmK)) * (nai / (nai + KmNa)) * rec_iNaK;
+      knak * (R)Dv[kKoOverKoKmK] * (nai / (nai + KmNa)) * rec_iNaK;
   const R ipca = gpca * cai / (KpCa + cai);
   const R rec_ipK = R(1.0) / (R(1.0) + fexp(divc(R(25) - u, R(5.98))));
   const R ipk = gpk * rec_ipK * (u - Ek);
@@ -653,8 +653,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   // calc_cass
   {
     const D CaSSBuf = Bufss * cassD / (cassD + Kbufss);
-    const D dCaSS = dtD * (-ixfer * (F64 ? Dv[kVcOverVss] : Vc / Vss) +
-                           irel * (F64 ? Dv[kVsrOverVss] : Vsr / Vss) +
+    const D dCaSS = dtD * (-ixfer * Dv[kVcOverVss] + irel * Dv[kVsrOverVss] +
                            -(D)ical * inversevssF2 * CAPACITANCE);
     const D bcss = Bufss - CaSSBuf - dCaSS - cassD + Kbufss;
     const D ccss = Kbufss * (CaSSBuf + dCaSS + cassD);
