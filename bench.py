@@ -264,6 +264,9 @@ def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False):
             for k, s in enumerate(specs)]
     srv = Server(classes, precision="fp32", max_tissues=64, chunk_steps=1000)
     stats = srv.serve(reqs, realtime=True)
+    shapes = {k: dict(variant=b.variant, tuning_ms={str(v): round(t, 4) for v, t in
+                                                   (b.tuning or {}).items()})
+              for k, b in srv.batches.items()}
     srv.close()
     cell_steps = sum(int(np.count_nonzero(r.sim.tissue.mesh == 1)) * r.sim.n_steps for r in reqs)
     out = dict(workload=f"{n_req} requests, Poisson 200/s, TP06 or Aliev-Panfilov 2D, side "
@@ -275,7 +278,8 @@ def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False):
                wall_s=round(stats["wall_s"], 2),
                gcell=round(cell_steps / stats["wall_s"] / 1e9, 3),
                max_concurrent=stats["max_concurrent"],
-               peak_hbm_mb=round(stats["peak_hbm_bytes"] / 2**20, 1))
+               peak_hbm_mb=round(stats["peak_hbm_bytes"] / 2**20, 1),
+               launch_shapes=shapes)
     if fp64_baseline:
         torch.cuda.synchronize()
         t_plain = 0.0

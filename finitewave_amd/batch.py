@@ -159,7 +159,10 @@ class TissueBatch:
     (``tune_steps`` each after one untimed step), keeps the fastest and
     remembers it per (device, model, precision, size class) in memory and in
     a JSON cache (``FW_AUTOTUNE_CACHE``, default
-    ``~/.cache/finitewave_amd/batch_tune.json``)."""
+    ``~/.cache/finitewave_amd/batch_tune.json``).  The size class is the
+    power of two of the batch's node count; admissions and retirements that
+    change it select the shape again (from the cache, or by timing), since
+    the best shape of a few small tissues is not that of many."""
 
     def __init__(self, model_cls, precision: str = "fp32", dt: float = 0.01, dr: float = 0.25,
                  max_tissues: int = 1024, use_graph: bool = True, chunk_cap: int = 4000,
@@ -202,6 +205,7 @@ class TissueBatch:
         self.autotune, self.tune_steps = autotune, tune_steps
         self.tuned = not autotune
         self.tuning: dict | None = None   # variant -> ms per step, when measured here
+        self._tuned_key: str | None = None  # size class the current shape was chosen for
         self.step = 0                     # global step of the next launch
         self.sims: dict[int, BatchSim] = {}
         self.buffers: dict[int, dict] = {}
@@ -368,12 +372,12 @@ class TissueBatch:
         key = self._tune_key()
         if key in _TUNE_CACHE:
             self.set_variant(_TUNE_CACHE[key])
-            self.tuned = True
+            self.tuned, self._tuned_key = True, key
             return 0
         vs = self.variants()
         per = self.tune_steps + 1
         if len(vs) < 2:
-            self.tuned = True
+            self.tuned, self._tuned_key = True, key
             return 0
         if n_avail < per * len(vs):
             return 0
@@ -402,7 +406,7 @@ class TissueBatch:
             best = min(near, key=timings.get)
         self.set_variant(best)
         self.tuning = timings
-        self.tuned = True
+        self.tuned, self._tuned_key = True, key
         _tune_cache_store(key, best, timings)
         return step - self.step
 
@@ -424,7 +428,7 @@ class TissueBatch:
             if ev:
                 target = min(target, sim.step0 + ev[0][0])
         n = max(1, target - self.step)
-        if not self.tuned:
+        if self.autotune and (not self.tuned or self._tune_key() != self._tuned_key):
             done = self._tune(n)
             self.step += done
             n -= done

@@ -264,3 +264,29 @@ def test_autotuner_picks_a_variant_and_keeps_results(gpu, tmp_path, monkeypatch)
     b.close()
     for s in sims:
         _check_same(s.result, _single(fw.MitchellSchaeffer2D, "fp32", s))
+
+
+@pytest.mark.gpu
+def test_autotuner_chooses_again_when_the_batch_grows(gpu, tmp_path, monkeypatch):
+    """The launch shape is chosen per size class (power of two of the node
+    count): a batch that grows from one small tissue to many chooses again,
+    and the results stay those of the single-tissue runs."""
+    from finitewave_amd import batch as B
+    monkeypatch.setenv("FW_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    monkeypatch.setattr(B, "_TUNE_CACHE", {})
+    sims = _sims("iso", n=6, seed=13, t_max=5.0)
+    b = TissueBatch(fw.MitchellSchaeffer2D, precision="fp32", max_tissues=8, tune_steps=4)
+    b.admit(sims[0])
+    while b.step < 200:
+        b.advance(200 - b.step)
+    first = b._tuned_key
+    assert b.tuned and first is not None
+    for s in sims[1:]:
+        b.admit(s)
+    b.advance()
+    assert b._tuned_key != first  # chosen again for the grown batch
+    b.run_all()  # shrinking back as tissues finish selects from the cache again
+    assert len(B._TUNE_CACHE) >= 2
+    b.close()
+    for s in sims:
+        _check_same(s.result, _single(fw.MitchellSchaeffer2D, "fp32", s))
