@@ -180,14 +180,15 @@ def ensemble_leg(n_tissues: int, steps: int, full: bool):
     t_adm = time.perf_counter() - t_adm
     b.advance(200)  # S1 at step 0, graph capture, warm-up
     torch.cuda.synchronize()
+    # events on the batch's own stream, the one its kernels are launched on
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    ev0.record()
+    ev0.record(b.stream)
     done = 0
     while done < steps:
         n0 = b.step
         b.advance(steps - done)
         done += b.step - n0
-    ev1.record()
+    ev1.record(b.stream)
     torch.cuda.synchronize()
     ms = ev0.elapsed_time(ev1)
     n_active = sum(int(np.count_nonzero(s.tissue.mesh == 1)) for s in sims)

@@ -150,6 +150,16 @@ def _tune_cache_store(key: str, variant: int, timings: dict) -> None:
         pass  # the cache is an optimisation; a read-only home does not stop the run
 
 
+def _on_stream(method):
+    """Runs a TissueBatch method with the batch's own stream current, so that
+    its allocations, copies and launches are ordered on that stream."""
+    def wrapped(self, *args, **kwargs):
+        with self.torch.cuda.stream(self.stream):
+            return method(self, *args, **kwargs)
+    wrapped.__name__, wrapped.__doc__ = method.__name__, method.__doc__
+    return wrapped
+
+
 class TissueBatch:
     """Many tissues of `model_cls` (a finitewave_amd 2D model class).
 
@@ -169,6 +179,9 @@ class TissueBatch:
                  autotune: bool = True, tune_steps: int = 16):
         self.torch = _native.require_gpu()
         self.dev = self.torch.device("cuda", self.torch.cuda.current_device())
+        # every batch works on a stream of its own (default priority): the
+        # batches of a server then overlap on the device
+        self.stream = self.torch.cuda.Stream(device=self.dev)
         self.proto = model_cls()
         self.proto.precision = precision
         self.prec = self.proto._precision_id()
@@ -233,6 +246,7 @@ class TissueBatch:
         return n * (2 * rb + st + 1 + 4) + rb * (horizon_steps + 1)
 
     # ------------------------------------------------------------------ admission
+    @_on_stream
     def admit(self, sim: BatchSim) -> int:
         if not self.free:
             raise RuntimeError("batch is full")
@@ -296,6 +310,7 @@ class TissueBatch:
         self.blocks_dirty = True
         return slot
 
+    @_on_stream
     def retire(self, slot: int) -> BatchSim:
         sim = self.sims.pop(slot)
         buf = self.buffers.pop(slot)
@@ -326,7 +341,7 @@ class TissueBatch:
 
     # ------------------------------------------------------------------ stepping
     def _stream(self):
-        return self.torch.cuda.current_stream().cuda_stream
+        return self.stream.cuda_stream
 
     def _rebuild_blocks(self) -> None:
         slots = np.array(sorted(self.sims), dtype=np.int32)
@@ -357,6 +372,7 @@ class TissueBatch:
     def variant(self) -> int:
         return lib().fw_batch_get_variant(self.handle)
 
+    @_on_stream
     def set_variant(self, variant: int) -> None:
         check(lib().fw_batch_set_variant(self.handle, int(variant), self._stream()),
               "fw_batch_set_variant")
@@ -410,6 +426,7 @@ class TissueBatch:
         _tune_cache_store(key, best, timings)
         return step - self.step
 
+    @_on_stream
     def advance(self, max_steps: int | None = None) -> list:
         """Run until the next admission point: at most `max_steps` steps (or
         the chunk cap), stopping early at stimulus steps and completions.
@@ -449,6 +466,7 @@ class TissueBatch:
 
     def close(self) -> None:
         if self.handle is not None and self.handle.value:
+            self.stream.synchronize()  # no launch of this batch still reads its tables
             lib().fw_batch_destroy(self.handle)
         self.handle = None
 

@@ -657,7 +657,9 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
     (void)fw_batch_destroy(b);
     return FW_E_NOMEM;
   }
-  if (hipMemset(b->table, 0, sizeof(BatchTissue) * max_tissues) != hipSuccess) {
+  // cleared before any stream (blocking or not) can use the batch
+  if (hipMemset(b->table, 0, sizeof(BatchTissue) * max_tissues) != hipSuccess ||
+      hipDeviceSynchronize() != hipSuccess) {
     set_error("fw_batch_create: table init failed");
     (void)fw_batch_destroy(b);
     return FW_E_HIP;

@@ -285,6 +285,8 @@ typedef struct fw_batch fw_batch;
 int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
                     double dr2, const double* params, int32_t n_params,
                     int32_t max_tissues, fw_batch** out);
+/* Frees the batch; no launch of it may still be pending (synchronize the
+ * stream its calls used first). */
 int fw_batch_destroy(fw_batch* batch);
 /* Writes slot's descriptor (host struct copied to the device table). */
 int fw_batch_set_tissue(fw_batch* batch, int32_t slot, const fw_tissue_desc* desc,
