@@ -54,8 +54,7 @@ EcgLauncher ecg_launcher(int precision, int model) {
   return table[precision][model];
 }
 
-void derive_params(int model, Params& P) {
-  if (model != 6) return;
+static void derive_tp06(Params& P) {
   // TP06 parameter order (tp06_2d.py:1073 call)
   const double ko = P.v[0], cao = P.v[1], nao = P.v[2], Vc = P.v[3], Vsr = P.v[4],
                Vss = P.v[5], R = P.v[24], F = P.v[25], T = P.v[26], KmK = P.v[34],
@@ -72,6 +71,11 @@ void derive_params(int model, Params& P) {
   d[kSqrtKo54] = sqrt(ko / 5.4);
   d[kVcOverVss] = Vc / Vss;
   d[kVsrOverVss] = Vsr / Vss;
+}
+
+void complete_params(int model, Params& P) {
+  if (model == 6) derive_tp06(P);
+  for (int q = 0; q < 64; ++q) P.f[q] = (float)P.v[q];  // round to nearest
 }
 
 __global__ void k_set_word(int32_t* p, int32_t v) { *p = v; }
@@ -188,7 +192,7 @@ int fw_ionic(int32_t model, int32_t precision, void* u_new, const void* u, void*
     }
   Params P;
   for (int q = 0; q < FW_MAX_PARAMS; ++q) P.v[q] = q < n_params ? params[q] : 0.0;
-  derive_params(model, P);
+  complete_params(model, P);
   return launch_ionic(model, precision, u_new, u, make_state(state, n_state), indexes,
                       n_indexes, dt, P, as_stream(stream));
 }
@@ -648,7 +652,7 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   const double mant = frexp(dr2, &e2);
   b->inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
   for (int q = 0; q < FW_MAX_PARAMS; ++q) b->P.v[q] = q < n_params ? params[q] : 0.0;
-  derive_params(model, b->P);
+  complete_params(model, b->P);
   b->host = new (std::nothrow) fw_tissue_desc[max_tissues]();
   if (!b->host || !b->active ||
       hipMalloc(&b->table, sizeof(BatchTissue) * max_tissues) != hipSuccess ||

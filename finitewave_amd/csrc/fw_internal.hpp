@@ -62,10 +62,11 @@ int launch_tips(int u_f64, const void* u_prev, const void* u, const int64_t* sha
                 double* tips, int32_t cap, int32_t* count, hipStream_t s);
 int launch_zero_f64(double* p, int64_t n, hipStream_t s);
 
-// Appends the model's parameter-only quantities after its parameters
-// (kTp06Derived for TP06); computed in fw_abi.hip, an FP64 unit built
-// without contraction, in the operation order of the expressions they replace.
-void derive_params(int model, Params& P);
+// Completes a parameter block filled with the model's parameters: appends
+// the parameter-only quantities (kTp06Derived for TP06), computed in
+// fw_abi.hip, an FP64 unit built without contraction, in the operation order
+// of the expressions they replace; then the float copies of every entry.
+void complete_params(int model, Params& P);
 
 // literal sweeps (fw_literal.hip)
 int launch_weights(int ndim, int kind, double* w, const int8_t* m,

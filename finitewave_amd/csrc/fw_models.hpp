@@ -24,8 +24,17 @@ constexpr int kBlock = 256;  // work-items per block of the node-parallel kernel
 constexpr int kXcds = 8;     // accelerator complex dies of an MI355X
 
 struct Params {
-  double v[64];
+  double v[64];  // the model's parameters (then derived quantities), as given
+  float f[64];   // the same rounded to float once on the host (complete_params)
 };
+
+// parameter q in the arithmetic type: the FP32 path reads the host's float
+// copy, a kernel-argument value, instead of converting per work-item
+template <typename R>
+__device__ __forceinline__ R par(const Params& P, int q) {
+  if constexpr (sizeof(R) == 4) return P.f[q];
+  else return P.v[q];
+}
 
 struct StatePtrs {
   void* p[24];
@@ -80,7 +89,7 @@ __device__ __forceinline__ R divc(R a, R b) {
 }
 
 // TP06 quantities that are the same for every node (functions of the
-// parameters only), appended after its 49 parameters by derive_params on the
+// parameters only), appended after its 49 parameters by complete_params on the
 // host, in the operation order of the expressions they replace: IEEE +, *, /
 // and sqrt round the same on host and device, so the FP64 path's values are
 // unchanged while each node saves ten divisions and a square root.  The FP32
@@ -181,8 +190,8 @@ __device__ __forceinline__ G rl_gate(G g, R inf, R tau, R dt) {
 template <typename R, typename SP>
 __device__ __forceinline__ R ionic_ap(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
-  const R a = (R)P.v[0], k = (R)P.v[1], eap = (R)P.v[2], mu_1 = (R)P.v[3],
-          mu_2 = (R)P.v[4];
+  const R a = par<R>(P, 0), k = par<R>(P, 1), eap = par<R>(P, 2), mu_1 = par<R>(P, 3),
+          mu_2 = par<R>(P, 4);
   auto&& vref = ref<R>(S, 0, c);
   R v = vref;
   // calc_v: v += (-dt*(eap + (mu_1*v)/(mu_2 + u)) * (v + k*u*(u - a - 1.)))
@@ -196,7 +205,7 @@ __device__ __forceinline__ R ionic_ap(R u, R un, SP& S, int64_t c,
 template <typename R, typename SP>
 __device__ __forceinline__ R ionic_barkley(R u, R un, SP& S,
                                            int64_t c, R dt, const Params& P) {
-  const R a = (R)P.v[0], b = (R)P.v[1], eap = (R)P.v[2];
+  const R a = par<R>(P, 0), b = par<R>(P, 1), eap = par<R>(P, 2);
   auto&& vref = ref<R>(S, 0, c);
   R v = vref;
   v += dt * (u - v);
@@ -208,8 +217,8 @@ __device__ __forceinline__ R ionic_barkley(R u, R un, SP& S,
 template <typename R, typename SP>
 __device__ __forceinline__ R ionic_ms(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
-  const R tau_close = (R)P.v[0], tau_open = (R)P.v[1], tau_in = (R)P.v[2],
-          tau_out = (R)P.v[3], u_gate = (R)P.v[4];
+  const R tau_close = par<R>(P, 0), tau_open = par<R>(P, 1), tau_in = par<R>(P, 2),
+          tau_out = par<R>(P, 3), u_gate = par<R>(P, 4);
   auto&& href = ref<R>(S, 0, c);
   R h = href;
   h += (u < u_gate) ? dt * (R(1) - h) / tau_open : dt * (-h) / tau_close;
@@ -224,10 +233,10 @@ __device__ __forceinline__ R ionic_ms(R u, R un, SP& S, int64_t c,
 template <typename R, typename SP>
 __device__ __forceinline__ R ionic_fk(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
-  const R tau_d = (R)P.v[0], tau_o = (R)P.v[1], tau_r = (R)P.v[2],
-          tau_si = (R)P.v[3], tau_v_m = (R)P.v[4], tau_v_p = (R)P.v[5],
-          tau_w_m = (R)P.v[6], tau_w_p = (R)P.v[7], k = (R)P.v[8],
-          u_c = (R)P.v[9], uc_si = (R)P.v[10];
+  const R tau_d = par<R>(P, 0), tau_o = par<R>(P, 1), tau_r = par<R>(P, 2),
+          tau_si = par<R>(P, 3), tau_v_m = par<R>(P, 4), tau_v_p = par<R>(P, 5),
+          tau_w_m = par<R>(P, 6), tau_w_p = par<R>(P, 7), k = par<R>(P, 8),
+          u_c = par<R>(P, 9), uc_si = par<R>(P, 10);
   auto&& vref = ref<R>(S, 0, c);
   auto&& wref = ref<R>(S, 1, c);
   R v = vref, w = wref;
@@ -248,16 +257,16 @@ __device__ __forceinline__ R ionic_fk(R u, R un, SP& S, int64_t c,
 template <typename R, typename SP>
 __device__ __forceinline__ R ionic_bo(R u, R un, SP& S, int64_t c,
                                       R dt, const Params& P) {
-  const R u_o = (R)P.v[0], u_u = (R)P.v[1], theta_v = (R)P.v[2],
-          theta_w = (R)P.v[3], theta_v_m = (R)P.v[4], theta_o = (R)P.v[5],
-          tau_v1_m = (R)P.v[6], tau_v2_m = (R)P.v[7], tau_v_p = (R)P.v[8],
-          tau_w1_m = (R)P.v[9], tau_w2_m = (R)P.v[10], k_w_m = (R)P.v[11],
-          u_w_m = (R)P.v[12], tau_w_p = (R)P.v[13], tau_fi = (R)P.v[14],
-          tau_o1 = (R)P.v[15], tau_o2 = (R)P.v[16], tau_so1 = (R)P.v[17],
-          tau_so2 = (R)P.v[18], k_so = (R)P.v[19], u_so = (R)P.v[20],
-          tau_s1 = (R)P.v[21], tau_s2 = (R)P.v[22], k_s = (R)P.v[23],
-          u_s = (R)P.v[24], tau_si = (R)P.v[25], tau_w_inf = (R)P.v[26],
-          w_inf_ = (R)P.v[27];
+  const R u_o = par<R>(P, 0), u_u = par<R>(P, 1), theta_v = par<R>(P, 2),
+          theta_w = par<R>(P, 3), theta_v_m = par<R>(P, 4), theta_o = par<R>(P, 5),
+          tau_v1_m = par<R>(P, 6), tau_v2_m = par<R>(P, 7), tau_v_p = par<R>(P, 8),
+          tau_w1_m = par<R>(P, 9), tau_w2_m = par<R>(P, 10), k_w_m = par<R>(P, 11),
+          u_w_m = par<R>(P, 12), tau_w_p = par<R>(P, 13), tau_fi = par<R>(P, 14),
+          tau_o1 = par<R>(P, 15), tau_o2 = par<R>(P, 16), tau_so1 = par<R>(P, 17),
+          tau_so2 = par<R>(P, 18), k_so = par<R>(P, 19), u_so = par<R>(P, 20),
+          tau_s1 = par<R>(P, 21), tau_s2 = par<R>(P, 22), k_s = par<R>(P, 23),
+          u_s = par<R>(P, 24), tau_si = par<R>(P, 25), tau_w_inf = par<R>(P, 26),
+          w_inf_ = par<R>(P, 27);
   auto&& vref = ref<R>(S, 0, c);
   auto&& wref = ref<R>(S, 1, c);
   auto&& sref = ref<R>(S, 2, c);
@@ -309,10 +318,10 @@ template <typename R, typename SP, typename U = R>
 __device__ __forceinline__ U ionic_lr91(U u_in, U un, SP& S, int64_t c, R dt,
                                         const Params& P) {
   const R u = (R)u_in;  // the potential in the arithmetic type
-  const R gna = (R)P.v[0], gsi = (R)P.v[1], gk = (R)P.v[2], gk1 = (R)P.v[3],
-          gkp = (R)P.v[4], gb = (R)P.v[5], ko = (R)P.v[6], ki = (R)P.v[7],
-          nai = (R)P.v[8], nao = (R)P.v[9], R_ = (R)P.v[11], T = (R)P.v[12],
-          F = (R)P.v[13], PR_NaK = (R)P.v[14];
+  const R gna = par<R>(P, 0), gsi = par<R>(P, 1), gk = par<R>(P, 2), gk1 = par<R>(P, 3),
+          gkp = par<R>(P, 4), gb = par<R>(P, 5), ko = par<R>(P, 6), ki = par<R>(P, 7),
+          nai = par<R>(P, 8), nao = par<R>(P, 9), R_ = par<R>(P, 11), T = par<R>(P, 12),
+          F = par<R>(P, 13), PR_NaK = par<R>(P, 14);
   // E_Na = R*T/F*np.log(nao/nai) (computed once per launch in the reference)
   const R E_Na = R_ * T / F * flog(nao / nai);
   auto&& mr = ref<R>(S, 0, c);
@@ -423,28 +432,28 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
                                         const Params& P) {
   const R u = (R)u_in;  // the potential in the arithmetic type
   typedef double D;
-  const R ko = (R)P.v[0], cao = (R)P.v[1], nao = (R)P.v[2];
+  const R ko = par<R>(P, 0), cao = par<R>(P, 1), nao = par<R>(P, 2);
   const D Vc = P.v[3], Vsr = P.v[4], Vss = P.v[5];
   const D Bufsr = P.v[8], Kbufsr = P.v[9], Bufss = P.v[10], Kbufss = P.v[11],
           Vmaxup = P.v[12], Kup = P.v[13], Vrel = P.v[14], k1_ = P.v[15],
           k2_ = P.v[16], k3 = P.v[17], k4 = P.v[18], EC = P.v[19],
           maxsr = P.v[20], minsr = P.v[21], Vleak = P.v[22], Vxfer = P.v[23];
-  const R R_ = (R)P.v[24], F = (R)P.v[25], T = (R)P.v[26], RTONF = (R)P.v[27];
+  const R R_ = par<R>(P, 24), F = par<R>(P, 25), T = par<R>(P, 26), RTONF = par<R>(P, 27);
   const D CAPACITANCE = P.v[28], Fd = P.v[25];
-  const R gkr = (R)P.v[29], pKNa = (R)P.v[30], gk1 = (R)P.v[31],
-          gna = (R)P.v[32], gbna = (R)P.v[33], KmK = (R)P.v[34],
-          KmNa = (R)P.v[35], knak = (R)P.v[36], gcal = (R)P.v[37],
-          gbca = (R)P.v[38], knaca = (R)P.v[39], KmNai = (R)P.v[40],
-          KmCa = (R)P.v[41], ksat = (R)P.v[42], n_ = (R)P.v[43],
-          gpca = (R)P.v[44], KpCa = (R)P.v[45], gpk = (R)P.v[46],
-          gto = (R)P.v[47], gks = (R)P.v[48];
+  const R gkr = par<R>(P, 29), pKNa = par<R>(P, 30), gk1 = par<R>(P, 31),
+          gna = par<R>(P, 32), gbna = par<R>(P, 33), KmK = par<R>(P, 34),
+          KmNa = par<R>(P, 35), knak = par<R>(P, 36), gcal = par<R>(P, 37),
+          gbca = par<R>(P, 38), knaca = par<R>(P, 39), KmNai = par<R>(P, 40),
+          KmCa = par<R>(P, 41), ksat = par<R>(P, 42), n_ = par<R>(P, 43),
+          gpca = par<R>(P, 44), KpCa = par<R>(P, 45), gpk = par<R>(P, 46),
+          gto = par<R>(P, 47), gks = par<R>(P, 48);
   constexpr bool F64 = sizeof(R) == 8;
-  const double* Dv = P.v + kTp06Derived;  // uniform quantities (derive_params)
+  const double* Dv = P.v + kTp06Derived;  // uniform quantities (complete_params)
   const D inverseVcF = Dv[kInvVcF];
   const D inversevssF2 = Dv[kInvVssF2];
   // x / (R T) for a node-dependent x
   auto overRT = [&](R x) -> R {
-    if constexpr (F64) return div_rn(x, (R)Dv[kRT], (R)Dv[kInvRT]);
+    if constexpr (F64) return div_rn(x, par<R>(P, kTp06Derived + kRT), par<R>(P, kTp06Derived + kInvRT));
     else return x / (R_ * T);
   };
 
@@ -544,11 +553,11 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     if constexpr (sizeof(R) == 4) {
       // (u - 15) / (e^X - 1) = 1 / ((2F/RT) * exprel(X)),  X = 2(u - 15)F/RT
       const R X = R(2) * (u - R(15)) * F / (R_ * T);
-      ical = gcal * d * f * f2 * fcass * R(4) * (R)Dv[kFFoverRT] *
+      ical = gcal * d * f * f2 * fcass * R(4) * par<R>(P, kTp06Derived + kFFoverRT) *
              (R(0.25) * fexp(X) * cass - cao) / (R(2) * F / (R_ * T) * exprel(X));
     } else {
       const R e = fexp(overRT(R(2) * (u - R(15)) * F));
-      ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * (R)Dv[kFFoverRT] *
+      ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * par<R>(P, kTp06Derived + kFFoverRT) *
              (R(0.25) * e * cass - cao) / (e - R(1.0));
     }
   }
@@ -585,7 +594,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     const R xr2 = xr2_inf - (xr2_inf - x2r) * fexp(-dt / tau_xr2);
     x1r = xr1;
     x2r = xr2;
-    ikr = gkr * (R)Dv[kSqrtKo54] * xr1 * xr2 * (u - Ek);
+    ikr = gkr * par<R>(P, kTp06Derived + kSqrtKo54) * xr1 * xr2 * (u - Ek);
   }
   // calc_iks
   R iks;
@@ -610,13 +619,13 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   const R e_n1 = fexp(overRT((n_ - R(1)) * u * F));
   const R inaca =
       knaca *
-      (R)Dv[kInvNaCaK] * (R)Dv[kInvKmCaCao] * (R(1.0) / (R(1) + ksat * e_n1)) *
+      par<R>(P, kTp06Derived + kInvNaCaK) * par<R>(P, kTp06Derived + kInvKmCaCao) * (R(1.0) / (R(1) + ksat * e_n1)) *
       (fexp(overRT(n_ * u * F)) * nai * nai * nai * cao - e_n1 * nao * nao * nao * cai * R(2.5));
   // calc_inak
   const R rec_iNaK = R(1.0) / (R(1.0) + R(0.1245) * fexp(overRT(R(-0.1) * u * F)) +
                                R(0.0353) * fexp(overRT(-u * F)));
   const R inak =
-      knak * (R)Dv[kKoOverKoKmK] * (nai / (nai + KmNa)) * rec_iNaK;
+      knak * par<R>(P, kTp06Derived + kKoOverKoKmK) * (nai / (nai + KmNa)) * rec_iNaK;
   const R ipca = gpca * cai / (KpCa + cai);
   const R rec_ipK = R(1.0) / (R(1.0) + fexp(divc(R(25) - u, R(5.98))));
   const R ipk = gpk * rec_ipK * (u - Ek);
@@ -686,18 +695,18 @@ template <typename R, typename SP, typename U = R>
 __device__ __forceinline__ U ionic_crn(U u_in, U un, SP& S, int64_t c, R dt,
                                         const Params& P) {
   const R u = (R)u_in;  // the potential in the arithmetic type
-  const R gna = (R)P.v[0], gnab = (R)P.v[1], gk1 = (R)P.v[2], gks = (R)P.v[4],
-          gto = (R)P.v[5], gcal = (R)P.v[6], gcab = (R)P.v[7],
-          gkur_coeff = (R)P.v[8], F = (R)P.v[9], T = (R)P.v[10],
-          R_ = (R)P.v[11], Vj = (R)P.v[13], Vup = (R)P.v[14],
-          Vrel = (R)P.v[15], ibk = (R)P.v[16], cao = (R)P.v[17],
-          nao = (R)P.v[18], ko = (R)P.v[19], caupmax = (R)P.v[20],
-          kup = (R)P.v[21], kmnai = (R)P.v[22], kmko = (R)P.v[23],
-          kmnancx = (R)P.v[24], kmcancx = (R)P.v[25], ksatncx = (R)P.v[26],
-          kmcmdn = (R)P.v[27], kmtrpn = (R)P.v[28], kmcsqn = (R)P.v[29],
-          trpnmax = (R)P.v[30], cmdnmax = (R)P.v[31], csqnmax = (R)P.v[32],
-          inacamax = (R)P.v[33], inakmax = (R)P.v[34], ipcamax = (R)P.v[35],
-          krel = (R)P.v[36], iupmax = (R)P.v[37], kq10 = (R)P.v[38];
+  const R gna = par<R>(P, 0), gnab = par<R>(P, 1), gk1 = par<R>(P, 2), gks = par<R>(P, 4),
+          gto = par<R>(P, 5), gcal = par<R>(P, 6), gcab = par<R>(P, 7),
+          gkur_coeff = par<R>(P, 8), F = par<R>(P, 9), T = par<R>(P, 10),
+          R_ = par<R>(P, 11), Vj = par<R>(P, 13), Vup = par<R>(P, 14),
+          Vrel = par<R>(P, 15), ibk = par<R>(P, 16), cao = par<R>(P, 17),
+          nao = par<R>(P, 18), ko = par<R>(P, 19), caupmax = par<R>(P, 20),
+          kup = par<R>(P, 21), kmnai = par<R>(P, 22), kmko = par<R>(P, 23),
+          kmnancx = par<R>(P, 24), kmcancx = par<R>(P, 25), ksatncx = par<R>(P, 26),
+          kmcmdn = par<R>(P, 27), kmtrpn = par<R>(P, 28), kmcsqn = par<R>(P, 29),
+          trpnmax = par<R>(P, 30), cmdnmax = par<R>(P, 31), csqnmax = par<R>(P, 32),
+          inacamax = par<R>(P, 33), inakmax = par<R>(P, 34), ipcamax = par<R>(P, 35),
+          krel = par<R>(P, 36), iupmax = par<R>(P, 37), kq10 = par<R>(P, 38);
   auto&& nair = ref<R>(S, 0, c);
   auto&& kir = ref<R>(S, 1, c);
   auto&& cair = ref<R>(S, 2, c);

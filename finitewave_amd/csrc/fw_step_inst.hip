@@ -70,7 +70,7 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
   a.dt = (Real)d.dt;
   a.s = (UReal)(d.D_model * d.dt / d.dr2);
   for (int q = 0; q < 64; ++q) a.P.v[q] = q < d.n_params ? d.params[q] : 0.0;
-  derive_params(d.model, a.P);
+  complete_params(d.model, a.P);
   a.act = tr.act;
   a.act_thr = (UReal)tr.act_thr;
   a.act_on = tr.act_on && tr.act != nullptr;
