@@ -675,6 +675,7 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
 int fw_batch_destroy(fw_batch* b) {
   if (!b) return FW_OK;
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  if (b->graph_long) (void)hipGraphExecDestroy(b->graph_long);
   if (b->table) (void)hipFree(b->table);
   if (b->btis) (void)hipFree(b->btis);
   if (b->bstart) (void)hipFree(b->bstart);
@@ -872,10 +873,10 @@ int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void
       return FW_E_NOMEM;
     }
     b->cap_blocks = (int32_t)cap;
-    if (b->graph) {
-      (void)hipGraphExecDestroy(b->graph);
-      b->graph = nullptr;
-    }
+    // the graphs captured the old table's addresses
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_long) (void)hipGraphExecDestroy(b->graph_long);
+    b->graph = b->graph_long = nullptr;
   }
   if (nblocks &&
       (hipMemcpyAsync(b->btis, ht, sizeof(int32_t) * nblocks, hipMemcpyHostToDevice, s) !=
@@ -895,6 +896,36 @@ int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void
   return FW_OK;
 }
 
+// Captures `pairs` pairs of batched steps into an executable graph; the step
+// words carry the step from one launch to the next, as for single tissues.
+static int capture_batch(fw_batch* b, BatchLauncher L, int32_t pairs, hipGraphExec_t* out) {
+  hipStream_t cap;
+  if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+    set_error("fw_batch_run: stream creation failed");
+    return FW_E_HIP;
+  }
+  hipGraph_t g = nullptr;
+  int rc = FW_OK;
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int32_t q = 0; q < pairs && !rc; ++q) {
+      rc = L(batch_args(b, 0, b->step_words, b->step_words + 1), cap);
+      if (!rc) rc = L(batch_args(b, 0, b->step_words + 1, b->step_words), cap);
+    }
+    e = hipStreamEndCapture(cap, &g);
+  }
+  hipGraphExec_t ex = nullptr;
+  if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess || rc || !ex) {
+    set_error("fw_batch_run: capture failed (%s)", hipGetErrorString(e));
+    return rc ? rc : FW_E_HIP;
+  }
+  *out = ex;
+  return FW_OK;
+}
+
 int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph, void* stream) {
   if (!b || n_steps < 0) {
     set_error("fw_batch_run: bad arguments");
@@ -905,44 +936,35 @@ int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph,
   hipStream_t s = as_stream(stream);
   int32_t done = 0;
   if (use_graph && n_steps >= 2) {
-    if (!b->graph || b->graph_nblocks != b->nblocks || b->graph_variant != b->variant) {
-      if (b->graph) {
-        (void)hipGraphExecDestroy(b->graph);
-        b->graph = nullptr;
-      }
-      hipStream_t cap;
-      if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
-        set_error("fw_batch_run: stream creation failed");
-        return FW_E_HIP;
-      }
-      hipGraph_t g = nullptr;
-      int rc = FW_OK;
-      hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
-      if (e == hipSuccess) {
-        rc = L(batch_args(b, 0, b->step_words, b->step_words + 1), cap);
-        if (!rc) rc = L(batch_args(b, 0, b->step_words + 1, b->step_words), cap);
-        e = hipStreamEndCapture(cap, &g);
-      }
-      hipGraphExec_t ex = nullptr;
-      if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      if (g) (void)hipGraphDestroy(g);
-      (void)hipStreamDestroy(cap);
-      if (e != hipSuccess || rc || !ex) {
-        set_error("fw_batch_run: capture failed (%s)", hipGetErrorString(e));
-        return rc ? rc : FW_E_HIP;
-      }
-      b->graph = ex;
+    const int32_t pairs = n_steps / 2;
+    if (b->graph_nblocks != b->nblocks || b->graph_variant != b->variant) {
+      // captured for another block table or launch shape
+      if (b->graph) (void)hipGraphExecDestroy(b->graph);
+      if (b->graph_long) (void)hipGraphExecDestroy(b->graph_long);
+      b->graph = b->graph_long = nullptr;
       b->graph_nblocks = b->nblocks;
       b->graph_variant = b->variant;
     }
+    if (!b->graph) {
+      int rc = capture_batch(b, L, 1, &b->graph);
+      if (rc) return rc;
+    }
+    if (pairs >= kGraphLongPairs && !b->graph_long) {
+      int rc = capture_batch(b, L, kGraphLongPairs, &b->graph_long);
+      if (rc) return rc;
+    }
     hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, b->step_words, (int32_t)step0);
-    const int32_t pairs = n_steps / 2;
-    for (int32_t q = 0; q < pairs; ++q) {
+    int32_t q = 0;
+    for (; q + kGraphLongPairs <= pairs; q += kGraphLongPairs)
+      if (hipGraphLaunch(b->graph_long, s) != hipSuccess) {
+        set_error("fw_batch_run: graph launch failed");
+        return FW_E_HIP;
+      }
+    for (; q < pairs; ++q)
       if (hipGraphLaunch(b->graph, s) != hipSuccess) {
         set_error("fw_batch_run: graph launch failed");
         return FW_E_HIP;
       }
-    }
     done = 2 * pairs;
   }
   for (int32_t q = done; q < n_steps; ++q) {

@@ -116,7 +116,8 @@ struct fw_batch {
   int32_t* bstart;
   int32_t nblocks, cap_blocks;
   int32_t* step_words;    // device int32[4]: graph step words [0..1], border flag [2]
-  hipGraphExec_t graph;
+  hipGraphExec_t graph;       // two steps
+  hipGraphExec_t graph_long;  // 2 * kGraphLongPairs steps, for long runs
   int32_t graph_nblocks, graph_variant;
   int32_t variant;        // launch shape of fw_batch_run
   int32_t* active;        // host: the occupied slots listed by fw_batch_set_active
