@@ -255,8 +255,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
         q.v[k] = need ? u[c + di * nj + dj] : U(0);
         q.m[k] = need ? (code[c + di * nj + dj] & 1) : 0;
       }
-    U val = diffuse9<U, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
-    val = ionic_update<MODEL, R>(uc, val, S, (int64_t)c, a.dt, a.P);
+    const U val = node_update<MODEL, R>(uc, S, (int64_t)c, a.dt, a.P, [&] {
+      return diffuse9<U, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+    });
     st_state<NT>(un + c, val);
   } else {
     const int32_t c = start + 4 * (int32_t)threadIdx.x;  // a multiple of 4
@@ -319,8 +320,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
         RegState<R, NS> rs;
 #pragma unroll
         for (int s = 0; s < NS; ++s) rs.v[s] = st[s][k];
-        R val = diffuse9<R, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
-        val = ionic_update<MODEL, R>(uc, val, rs, 0, a.dt, a.P);
+        const R val = node_update<MODEL, R>(uc, rs, 0, a.dt, a.P, [&] {
+          return diffuse9<R, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+        });
         out[k] = upd ? val : uc;
 #pragma unroll
         for (int s = 0; s < NS; ++s) st[s][k] = upd ? rs.v[s] : st[s][k];
@@ -353,8 +355,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<
       RegState<R, NS> rs;
 #pragma unroll
       for (int s = 0; s < NS; ++s) rs.v[s] = ld_state<NT>((const global_t<R>*)S.p[s] + cc);
-      R val = diffuse9<R, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
-      val = ionic_update<MODEL, R>(uc, val, rs, 0, a.dt, a.P);
+      const R val = node_update<MODEL, R>(uc, rs, 0, a.dt, a.P, [&] {
+        return diffuse9<R, EXACT>(stencil, dd, df, q, a.D, a.dtd, a.dr2, a.inv_dr2, a.s);
+      });
       st_state<NT>(un + cc, val);
 #pragma unroll
       for (int s = 0; s < NS; ++s) st_state<NT>((global_t<R>*)S.p[s] + cc, rs.v[s]);

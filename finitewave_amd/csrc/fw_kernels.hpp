@@ -642,23 +642,9 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void
     bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
     if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
     if (interior) {
-      if constexpr (PREC == 0) {
-        // Every model's update ends in un + X or un - X with X free of un.
-        // Built without contraction, forming it first with un = 0 gives X
-        // (or -X) exactly, and adding the diffusion result last gives the
-        // value of the reference's diffusion-then-ionic order, while the
-        // neighbour values and the diffusion result no longer stay live
-        // across the ionic update (the 2D TP06 step then fits 128 registers
-        // unspilled).  The FP32 path keeps the literal order: with
-        // contraction the compiler may fuse the final addition into the
-        // diffusion's multiply-add, and the batched kernel must match it.
-        const U r = ionic_update<MODEL, R>(uc, U(0), a.st, c, a.dt, a.P);
-        a.u_new[c] = diffuse<NDIM, ST, R, U, true>(a, i, j, k, c, uc) + r;
-      } else {
-        U un = diffuse<NDIM, ST, R, U, false>(a, i, j, k, c, uc);
-        un = ionic_update<MODEL, R>(uc, un, a.st, c, a.dt, a.P);
-        a.u_new[c] = un;
-      }
+      a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
+        return diffuse<NDIM, ST, R, U, PREC == 0>(a, i, j, k, c, uc);
+      });
     }
   }
   if ((code & 4) && a.probe_row >= 0) {
