@@ -979,17 +979,23 @@ __host__ __device__ constexpr int model_n_state(int model) {
        : model == 4 ? 3 : model == 5 ? 7 : model == 6 ? 19 : 21;
 }
 // u_new of an updated node.  Every model's update ends in un + X or un - X
-// with X free of un, so the update is formed first with un = 0 and the
-// diffusion result (`diffusion()`, evaluated afterwards) is added last: the
-// neighbour values and the diffusion result then do not stay live across
-// the ionic code.  Without contraction (FP64) the value is the reference's
-// diffusion-then-ionic one exactly; with contraction (FP32) the single-tissue
-// and batched kernels both use this function, so they stay bit-identical.
+// with X free of un.  For the models with long ionic updates (LR91, TP06,
+// Courtemanche) the update is formed first with un = 0 and the diffusion
+// result (`diffusion()`, evaluated afterwards) is added last, so the
+// neighbour values and the diffusion result do not stay live across the
+// ionic code; the light models keep the literal order, which hides the
+// neighbour loads better.  Without contraction (FP64) both orders give the
+// reference's value exactly; with contraction (FP32) the single-tissue and
+// batched kernels both use this function, so they stay bit-identical.
 template <int MODEL, typename R, typename U, typename SP, typename Diffusion>
 __device__ __forceinline__ U node_update(U uc, SP& S, int64_t c, R dt, const Params& P,
                                          Diffusion&& diffusion) {
-  const U r = ionic_update<MODEL, R>(uc, U(0), S, c, dt, P);
-  return diffusion() + r;
+  if constexpr (MODEL >= 5) {
+    const U r = ionic_update<MODEL, R>(uc, U(0), S, c, dt, P);
+    return diffusion() + r;
+  } else {
+    return ionic_update<MODEL, R>(uc, diffusion(), S, c, dt, P);
+  }
 }
 
 __host__ __device__ constexpr int model_n_params(int model) {
