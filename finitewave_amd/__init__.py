@@ -32,6 +32,8 @@ from .tracker import (Tracker, TrackerSequence, ActivationTime2DTracker,
                       SpiralWaveCore2DTracker, SpiralWaveCore3DTracker, ECG2DTracker,
                       ECG3DTracker)
 from .command import Command, CommandSequence
+from .fibrosis import (FibrosisPattern, Diffuse2DPattern, Structural2DPattern,
+                       Diffuse3DPattern, Structural3DPattern)
 from .state import StateSaver, StateSaverCollection, StateLoader
 from ._native import NativeUnavailableError
 
