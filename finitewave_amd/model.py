@@ -371,9 +371,13 @@ class CardiacModel:
             setattr(m, name, t.cpu().numpy())
 
     def _after_host_change(self, eng) -> None:
-        """A command may rewrite state, mesh, fibers or weights."""
+        """A command may rewrite state, mesh, fibers or weights.  A command
+        that calls ``compute_weights()`` (the reference's way to make a mesh
+        change take effect) drops the model's engine; the running loop's
+        engine is rebuilt from the new mesh and kept as the model's."""
         eng.build()
         eng.upload()
+        self._engine = eng
 
     def _collect_probes(self, probes, probe_trackers, slices, owners) -> None:
         data = probes.buf[: len(owners)].cpu().numpy()

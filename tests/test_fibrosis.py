@@ -203,3 +203,54 @@ def test_interstitial_fibrosis_run_matches_oracle(gpu):
     sim.run(10)
     assert np.array_equal(model.u, sim.u)
     assert np.array_equal(model.v, sim.state["v"])
+
+
+@pytest.mark.gpu
+def test_fibrosis_added_by_a_command_mid_run_matches_oracle(gpu):
+    """A command at t = 4 writes a structural pattern into the mesh and calls
+    compute_weights(); the wave then meets the new obstacle.  Bit for bit
+    the oracle's run with the same command, and a second run() continues on
+    the new mesh."""
+    from oracle.oracle import OCommand, OracleSim, OStim
+    n = 60
+    random.seed(7)
+    block = fw.Structural2DPattern(0.5, 3, 3, 20, 44, 10, 50).generate(shape=(n, n))
+
+    class AddFibrosis(fw.Command):
+        def execute(self, model):
+            model.cardiac_tissue.mesh = np.where(block == 2, 2, model.cardiac_tissue.mesh
+                                                 ).astype(np.int8)
+            model.compute_weights()
+
+    tissue = fw.CardiacTissue2D([n, n])
+    model = fw.AlievPanfilov2D()
+    model.dt, model.dr, model.t_max = 0.01, 0.25, 10
+    model.cardiac_tissue = tissue
+    model.prog_bar = False
+    seq = fw.StimSequence()
+    seq.add_stim(fw.StimVoltageCoord2D(0, 1.0, 0, n, 0, 4))
+    model.stim_sequence = seq
+    cs = fw.CommandSequence()
+    cs.add_command(AddFibrosis(4))
+    model.command_sequence = cs
+    model.run()
+    assert np.any(model.cardiac_tissue.mesh == 2)
+
+    mesh0 = fw.CardiacTissue2D([n, n]).mesh
+
+    def add(sim):
+        sim.mesh = np.where(block == 2, 2, sim.mesh).astype(np.int8)
+        sim.mesh[0, :] = sim.mesh[-1, :] = sim.mesh[:, 0] = sim.mesh[:, -1] = 0
+        sim.compute_weights()
+
+    sim = OracleSim("aliev_panfilov", mesh0, 0.01, 0.25)
+    sim.stims.append(OStim("voltage", 0, 1.0, (slice(0, n), slice(0, 4))))
+    sim.commands.append(OCommand(4, add))
+    sim.run(10)
+    assert np.array_equal(sim.mesh, model.cardiac_tissue.mesh)
+    assert np.array_equal(model.u, sim.u)
+    assert np.array_equal(model.v, sim.state["v"])
+    model.t_max = 14
+    model.run(initialize=False)
+    sim.run(14, initialize=False)
+    assert np.array_equal(model.u, sim.u)
