@@ -2,7 +2,10 @@
 
 One translation unit per (precision, model) pair for the fused step kernel,
 compiled in parallel: FP64 units with ``-ffp-contract=off`` (no fused
-multiply-add, the reference's arithmetic), FP32 units with contraction.
+multiply-add, the reference's arithmetic), FP32 units with ``-ffp-contract=on``:
+multiply-adds fused within one source expression only, so every kernel that
+inlines a formula rounds it the same way (the backend's cross-statement
+fusion under ``fast`` depends on each kernel's basic blocks).
 Packed FP32 instructions are disabled in every unit.
 """
 from __future__ import annotations
@@ -24,7 +27,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", str(CSRC)]
 FP64 = ["-ffp-contract=off"]
-FP32 = ["-ffp-contract=fast", "-fno-math-errno", "-freciprocal-math", "-fapprox-func",
+FP32 = ["-ffp-contract=on", "-fno-math-errno", "-freciprocal-math", "-fapprox-func",
         "-fno-signed-zeros",
         "-fno-trapping-math"]
 

@@ -70,10 +70,7 @@ struct BatchArgs {
 // 3x3 neighbourhood of one node, row-major: NW N NE / W C E / SW S SE
 // (rows are i-1, i, i+1; columns j-1, j, j+1): potentials and conduct bits
 template <typename R>
-struct Nb9 {
-  R v[9];
-  int m[9];
-};
+using Nb9 = Hood2<R>;
 
 // diffused u at an interior node of a uniform-field tissue: FP64 exact
 // weights (the reference's weight arithmetic and summation order, EXACT) or
@@ -99,12 +96,7 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
     } else {
       const R dx = sizeof(R) == 8 ? (R)dd[0] : (R)df[0];
       const R dy = sizeof(R) == 8 ? (R)dd[1] : (R)df[1];
-      R fl = R(0);
-      fl += dx * face_w<R>(mN, mS) * (q.v[1] - uc);
-      fl += dy * face_w<R>(mW, mE) * (q.v[3] - uc);
-      fl += dy * face_w<R>(mE, mW) * (q.v[5] - uc);
-      fl += dx * face_w<R>(mS, mN) * (q.v[7] - uc);
-      return uc + s * fl;
+      return flux_iso<R>(q, dx, dy, dy, dx, s);
     }
   }
   Nbr M;
@@ -126,22 +118,7 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
     const R dxy = sizeof(R) == 8 ? (R)dd[1] : (R)df[1];
     const R dyx = sizeof(R) == 8 ? (R)dd[2] : (R)df[2];
     const R dyy = sizeof(R) == 8 ? (R)dd[3] : (R)df[3];
-    const R uNW = q.v[0], uN = q.v[1], uNE = q.v[2];
-    const R uW = q.v[3], uE = q.v[5];
-    const R uSW = q.v[6], uS = q.v[7], uSE = q.v[8];
-    const int mNW = q.m[0], mN = q.m[1], mNE = q.m[2];
-    const int mW = q.m[3], mE = q.m[5];
-    const int mSW = q.m[6], mS = q.m[7], mSE = q.m[8];
-    R fl = R(0);
-    fl += dxx * (R)mN * (uN - uc);
-    fl -= minor_flux<R>(dxy, mNW, uNW, mW, uW, mN, uN, 1, uc, mNE, uNE, mE, uE);
-    fl += dyy * (R)mW * (uW - uc);
-    fl -= minor_flux<R>(dyx, mNW, uNW, mN, uN, mW, uW, 1, uc, mSW, uSW, mS, uS);
-    fl += dyy * (R)mE * (uE - uc);
-    fl += minor_flux<R>(dyx, mNE, uNE, mN, uN, mE, uE, 1, uc, mSE, uSE, mS, uS);
-    fl += dxx * (R)mS * (uS - uc);
-    fl += minor_flux<R>(dxy, mSW, uSW, mW, uW, mS, uS, 1, uc, mSE, uSE, mE, uE);
-    return uc + s * fl;
+    return flux_aniso<R>(q, dxx, dxx, dxy, dxy, dyx, dyx, dyy, dyy, s);
   }
 }
 

@@ -49,6 +49,13 @@ __device__ __forceinline__ F fval(const Field& f, int64_t i, int64_t j, int64_t 
   else return ((const global_t<double>*)f.data)[x];
 }
 
+// a uniform field's value in the type F the diffusion is computed in
+template <typename F>
+__device__ __forceinline__ F funi(const Field& f) {
+  if constexpr (sizeof(F) == 4) return f.fvalue;
+  else return f.value;
+}
+
 // a field at node c and at c's lower neighbour along axis DIR (flat stride
 // `step`): one load when the field is uniform or tabulated along another axis
 template <typename F, int DIR>
@@ -112,6 +119,7 @@ struct KArgs {
   int32_t probe_row;
   int32_t idx32;      // N < 2^32: node coordinates by 32-bit division
   UDiv div_jk, div_k;  // division by nj * nk and by nk
+  int32_t uni_fields;  // every field uniform (mode 0)
 };
 
 // (i, j, k) of flat node index c in a C-order grid (k = 0 in 2D)
@@ -321,12 +329,38 @@ __device__ __forceinline__ void sym_comp(double dxx, double dxy, double dyx,
 // FP32 flux form
 // ---------------------------------------------------------------------------
 
-// d * (mean of conducting higher-side nodes - mean of lower-side nodes); the
+// (mean of conducting higher-side nodes - mean of lower-side nodes), or 0
+// when the face's minor weights are zero (the reference's rule): times d, the
 // algebraic value of sum_k q_k u_k for the minor weights q of the reference.
+// Callers add it as fl = fma(+-d, minor_diff(...), fl); with the face off
+// that adds d * 0 = 0 exactly.  The one multiply-add inside is written as
+// fma, with contraction off, so its rounding is the same in every kernel
+// that inlines it.
 template <typename R>
-__device__ __forceinline__ R minor_flux(R d, int m0, R u0, int m1, R u1, int m2,
-                                        R u2, int m3, R u3, int m4, R u4,
-                                        int m5, R u5) {
+__device__ __forceinline__ R minor_diff(int m0, R u0, int m1, R u1, int m2, R u2, int m3,
+                                        R u3, int m4, R u4, int m5, R u5) {
+#pragma clang fp contract(off)
+  const int mh = m2 + m3 + m4 + m5;
+  const int ml = m0 + m1 + m2 + m3;
+  const bool on = m2 != 0 && m3 != 0 && mh >= 3 && ml >= 3;
+  const R common = u2 + u3;
+  const R sh = common + (m4 ? u4 : R(0)) + (m5 ? u5 : R(0));
+  const R sl = common + (m0 ? u0 : R(0)) + (m1 ? u1 : R(0));
+  const R ih = mh == 4 ? R(0.25) : R(1.0 / 3.0);
+  const R il = ml == 4 ? R(0.25) : R(1.0 / 3.0);
+  const R v = fma(sh, ih, -(sl * il));
+  return on ? v : R(0);  // a select: no branch around the neighbour loads
+}
+
+// d * minor_diff(...) with the select last, the form of the 3D flux: there
+// it keeps the 19-point FP32 step at 71 registers (7 waves per SIMD) instead
+// of 84 (5).  The 3D step has no batched counterpart to agree with, so it
+// lets the compiler fuse across statements (the masked minor flux folds into
+// one fma with the running sum).
+template <typename R>
+__device__ __forceinline__ R minor_flux(R d, int m0, R u0, int m1, R u1, int m2, R u2, int m3,
+                                        R u3, int m4, R u4, int m5, R u5) {
+#pragma clang fp contract(fast)
   const int mh = m2 + m3 + m4 + m5;
   const int ml = m0 + m1 + m2 + m3;
   const bool on = m2 != 0 && m3 != 0 && mh >= 3 && ml >= 3;
@@ -336,7 +370,7 @@ __device__ __forceinline__ R minor_flux(R d, int m0, R u0, int m1, R u1, int m2,
   const R ih = mh == 4 ? R(0.25) : R(1.0 / 3.0);
   const R il = ml == 4 ? R(0.25) : R(1.0 / 3.0);
   const R v = d * (sh * ih - sl * il);
-  return on ? v : R(0);  // a select: no branch around the neighbour loads
+  return on ? v : R(0);
 }
 
 // FP32 face weight factor, branch-free: 0 when the neighbour does not
@@ -358,6 +392,108 @@ __device__ __forceinline__ double state_as_double(const StatePtrs& S, int k,
                                                   int64_t c, int prec) {
   if (state_is_f64(MODEL, prec, k)) return static_cast<const double*>(S.p[k])[c];
   return (double)static_cast<const float*>(S.p[k])[c];
+}
+
+// The 3x3 neighbourhood of an interior 2D node: potentials and conduct bits
+// (row-major, centre at 4; the isotropic stencil leaves the corners 0).  The
+// flux form takes it as an argument so that the fused step can load it
+// before the ionic update and the loads' latency overlaps that arithmetic.
+template <typename R>
+struct Hood2 {
+  R v[9];
+  int m[9];
+};
+
+template <int ST, typename RA, typename R>
+__device__ __forceinline__ Hood2<R> load_hood2(const KArgs<RA, R>& a, int64_t c, R uc) {
+  Hood2<R> q;
+  const int64_t sN = a.nj;
+#pragma unroll
+  for (int di = -1; di <= 1; ++di)
+#pragma unroll
+    for (int dj = -1; dj <= 1; ++dj) {
+      const int k = (di + 1) * 3 + dj + 1;
+      const bool need = (ST == 1 || di == 0 || dj == 0) && k != 4;
+      q.v[k] = need ? a.u[c + di * sN + dj] : R(0);
+      q.m[k] = need ? (a.code[c + di * sN + dj] & 1) : 0;
+    }
+  q.v[4] = uc;
+  q.m[4] = 1;
+  return q;
+}
+
+// The flux-form arithmetic of the 2D stencils, shared by the single-tissue
+// and the batched step.  Contraction is off here and the multiply-adds are
+// written as fma: the rounding then does not depend on the fusion choices
+// the compiler makes in each kernel's context, and both kernels give the
+// same bits.  Face values: 0 = the face towards lower i (x) or j (y).
+template <typename R>
+__device__ __forceinline__ R flux_iso(const Hood2<R>& q, R dN, R dW, R dE, R dS, R s) {
+#pragma clang fp contract(off)
+  const R uc = q.v[4];
+  const int mN = q.m[1], mW = q.m[3], mE = q.m[5], mS = q.m[7];
+  R fl = (dN * face_w<R>(mN, mS)) * (q.v[1] - uc);
+  fl = fma(dW * face_w<R>(mW, mE), q.v[3] - uc, fl);
+  fl = fma(dE * face_w<R>(mE, mW), q.v[5] - uc, fl);
+  fl = fma(dS * face_w<R>(mS, mN), q.v[7] - uc, fl);
+  return fma(s, fl, uc);
+}
+
+template <typename R>
+__device__ __forceinline__ R flux_aniso(const Hood2<R>& q, R dxx0, R dxx1, R dxy0, R dxy1,
+                                        R dyx0, R dyx1, R dyy0, R dyy1, R s) {
+#pragma clang fp contract(off)
+  const R uNW = q.v[0], uN = q.v[1], uNE = q.v[2], uW = q.v[3], uc = q.v[4], uE = q.v[5];
+  const R uSW = q.v[6], uS = q.v[7], uSE = q.v[8];
+  const int mNW = q.m[0], mN = q.m[1], mNE = q.m[2], mW = q.m[3], mE = q.m[5];
+  const int mSW = q.m[6], mS = q.m[7], mSE = q.m[8];
+  // face x0 (i-1/2): major, then the minor with sign -
+  R fl = (dxx0 * (R)mN) * (uN - uc);
+  fl = fma(-dxy0, minor_diff<R>(mNW, uNW, mW, uW, mN, uN, 1, uc, mNE, uNE, mE, uE), fl);
+  // face y0 (j-1/2)
+  fl = fma(dyy0 * (R)mW, uW - uc, fl);
+  fl = fma(-dyx0, minor_diff<R>(mNW, uNW, mN, uN, mW, uW, 1, uc, mSW, uSW, mS, uS), fl);
+  // face y1 (j+1/2)
+  fl = fma(dyy1 * (R)mE, uE - uc, fl);
+  fl = fma(dyx1, minor_diff<R>(mNE, uNE, mN, uN, mE, uE, 1, uc, mSE, uSE, mS, uS), fl);
+  // face x1 (i+1/2)
+  fl = fma(dxx1 * (R)mS, uS - uc, fl);
+  fl = fma(dxy1, minor_diff<R>(mSW, uSW, mW, uW, mS, uS, 1, uc, mSE, uSE, mE, uE), fl);
+  return fma(s, fl, uc);
+}
+
+// FP32 flux form of the 2D isotropic (ST 0) and anisotropic (ST 1) stencils
+// on a loaded neighbourhood; R is the potential's storage type.  UNI: every
+// field is uniform and its value comes straight from the arguments
+template <int ST, bool UNI, typename RA, typename R>
+__device__ __forceinline__ R flux2d(const KArgs<RA, R>& a, int64_t i, int64_t j, int64_t c,
+                                    R uc, const Hood2<R>& q) {
+  const int64_t sN = a.nj;
+  if constexpr (ST == 0) {
+    R dN, dS, dW, dE;
+    if constexpr (UNI) {
+      dN = dS = funi<R>(a.f[0]);
+      dW = dE = funi<R>(a.f[1]);
+    } else {
+      fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dS, dN);
+      fval_pair<R, 1>(a.f[1], i, j, 0, c, 1, dE, dW);
+    }
+    return flux_iso<R>(q, dN, dW, dE, dS, a.s);
+  } else {
+    R dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1;
+    if constexpr (UNI) {
+      dxx0 = dxx1 = funi<R>(a.f[0]);
+      dxy0 = dxy1 = funi<R>(a.f[1]);
+      dyx0 = dyx1 = funi<R>(a.f[2]);
+      dyy0 = dyy1 = funi<R>(a.f[3]);
+    } else {
+      fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dxx1, dxx0);
+      fval_pair<R, 0>(a.f[1], i, j, 0, c, sN, dxy1, dxy0);
+      fval_pair<R, 1>(a.f[2], i, j, 0, c, 1, dyx1, dyx0);
+      fval_pair<R, 1>(a.f[3], i, j, 0, c, 1, dyy1, dyy0);
+    }
+    return flux_aniso<R>(q, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1, a.s);
+  }
 }
 
 // R here is the potential's storage type (FP32 path of the millivolt-scale
@@ -384,32 +520,26 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
                  (double)uc * W[4] + (double)u[c + 1] * W[5] +
                  (double)u[c + sN - 1] * W[6] + (double)u[c + sN] * W[7] +
                  (double)u[c + sN + 1] * W[8]);
+    } else if constexpr (!EXACT) {
+      const Hood2<R> q = load_hood2<ST>(a, c, uc);
+      return a.uni_fields ? flux2d<ST, true>(a, i, j, c, uc, q)
+                          : flux2d<ST, false>(a, i, j, c, uc, q);
     } else if constexpr (ST == 0) {
       const int mN = a.code[c - sN] & 1, mS = a.code[c + sN] & 1;
       const int mW = a.code[c - 1] & 1, mE = a.code[c + 1] & 1;
       R dN, dS, dW, dE;
       fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dS, dN);
       fval_pair<R, 1>(a.f[1], i, j, 0, c, 1, dE, dW);
-      if constexpr (EXACT) {
-        double W[5];
-        W[0] = iso_comp(dN, mN, mS);
-        W[1] = iso_comp(dW, mW, mE);
-        W[3] = iso_comp(dE, mE, mW);
-        W[4] = iso_comp(dS, mS, mN);
-        W[2] = -(W[0] + W[1] + W[3] + W[4]);
-        scale_exact<5>(W, 2, a.D, a.dtd, a.dr2, a.inv_dr2);
-        return u[c - sN] * W[0] + u[c - 1] * W[1] + uc * W[2] + u[c + 1] * W[3] +
-               u[c + sN] * W[4];
-      } else {
-        const R uN = u[c - sN], uW = u[c - 1], uE = u[c + 1], uS = u[c + sN];
-        R fl = R(0);
-        fl += (R)dN * face_w<R>(mN, mS) * (uN - uc);
-        fl += (R)dW * face_w<R>(mW, mE) * (uW - uc);
-        fl += (R)dE * face_w<R>(mE, mW) * (uE - uc);
-        fl += (R)dS * face_w<R>(mS, mN) * (uS - uc);
-        return uc + a.s * fl;
-      }
-    } else {  // anisotropic 9-point
+      double W[5];
+      W[0] = iso_comp(dN, mN, mS);
+      W[1] = iso_comp(dW, mW, mE);
+      W[3] = iso_comp(dE, mE, mW);
+      W[4] = iso_comp(dS, mS, mN);
+      W[2] = -(W[0] + W[1] + W[3] + W[4]);
+      scale_exact<5>(W, 2, a.D, a.dtd, a.dr2, a.inv_dr2);
+      return u[c - sN] * W[0] + u[c - 1] * W[1] + uc * W[2] + u[c + 1] * W[3] +
+             u[c + sN] * W[4];
+    } else {  // anisotropic 9-point, exact weights
       Nbr M;
 #pragma unroll
       for (int di = -1; di <= 1; ++di)
@@ -424,35 +554,12 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
       fval_pair<R, 0>(a.f[1], i, j, 0, c, sN, dxy1, dxy0);
       fval_pair<R, 1>(a.f[2], i, j, 0, c, 1, dyx1, dyx0);
       fval_pair<R, 1>(a.f[3], i, j, 0, c, 1, dyy1, dyy0);
-      if constexpr (EXACT) {
-        double W[9];
-        wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
-        scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
-        return u[c - sN - 1] * W[0] + u[c - sN] * W[1] + u[c - sN + 1] * W[2] +
-               u[c - 1] * W[3] + uc * W[4] + u[c + 1] * W[5] +
-               u[c + sN - 1] * W[6] + u[c + sN] * W[7] + u[c + sN + 1] * W[8];
-      } else {
-        const R uNW = u[c - sN - 1], uN = u[c - sN], uNE = u[c - sN + 1];
-        const R uW = u[c - 1], uE = u[c + 1];
-        const R uSW = u[c + sN - 1], uS = u[c + sN], uSE = u[c + sN + 1];
-        const int mNW = M(-1, -1, 0), mN = M(-1, 0, 0), mNE = M(-1, 1, 0);
-        const int mW = M(0, -1, 0), mE = M(0, 1, 0);
-        const int mSW = M(1, -1, 0), mS = M(1, 0, 0), mSE = M(1, 1, 0);
-        R fl = R(0);
-        // face x0 (i-1/2): major, then the minor with sign -
-        fl += (R)dxx0 * (R)mN * (uN - uc);
-        fl -= minor_flux<R>((R)dxy0, mNW, uNW, mW, uW, mN, uN, 1, uc, mNE, uNE, mE, uE);
-        // face y0 (j-1/2)
-        fl += (R)dyy0 * (R)mW * (uW - uc);
-        fl -= minor_flux<R>((R)dyx0, mNW, uNW, mN, uN, mW, uW, 1, uc, mSW, uSW, mS, uS);
-        // face y1 (j+1/2)
-        fl += (R)dyy1 * (R)mE * (uE - uc);
-        fl += minor_flux<R>((R)dyx1, mNE, uNE, mN, uN, mE, uE, 1, uc, mSE, uSE, mS, uS);
-        // face x1 (i+1/2)
-        fl += (R)dxx1 * (R)mS * (uS - uc);
-        fl += minor_flux<R>((R)dxy1, mSW, uSW, mW, uW, mS, uS, 1, uc, mSE, uSE, mE, uE);
-        return uc + a.s * fl;
-      }
+      double W[9];
+      wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
+      scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
+      return u[c - sN - 1] * W[0] + u[c - sN] * W[1] + u[c - sN + 1] * W[2] +
+             u[c - 1] * W[3] + uc * W[4] + u[c + 1] * W[5] +
+             u[c + sN - 1] * W[6] + u[c + sN] * W[7] + u[c + sN + 1] * W[8];
     }
   } else {  // NDIM == 3
     if constexpr (ST == 2) {
@@ -540,6 +647,7 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
                u[c - si - 1] * W[15] + u[c + si - 1] * W[16] +
                u[c - si + 1] * W[17] + u[c + si + 1] * W[18];
       } else {
+#pragma clang fp contract(fast)
         // the 18 neighbours, loaded unconditionally; U() masks them
         R un18[27];
 #pragma unroll
@@ -559,51 +667,51 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
         // x0
         fl += (R)d[0][0] * (R)M(-1, 0, 0) * (U(-1, 0, 0) - uc);
         fl -= minor_flux<R>((R)d[0][1], M(-1, -1, 0), U(-1, -1, 0), M(0, -1, 0), U(0, -1, 0),
-                            M(-1, 0, 0), U(-1, 0, 0), 1, uc, M(-1, 1, 0), U(-1, 1, 0),
-                            M(0, 1, 0), U(0, 1, 0));
+                            M(-1, 0, 0), U(-1, 0, 0), 1, uc,
+                            M(-1, 1, 0), U(-1, 1, 0), M(0, 1, 0), U(0, 1, 0));
         fl -= minor_flux<R>((R)d[0][2], M(-1, 0, -1), U(-1, 0, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(-1, 0, 0), U(-1, 0, 0), 1, uc, M(-1, 0, 1), U(-1, 0, 1),
-                            M(0, 0, 1), U(0, 0, 1));
+                            M(-1, 0, 0), U(-1, 0, 0), 1, uc,
+                            M(-1, 0, 1), U(-1, 0, 1), M(0, 0, 1), U(0, 0, 1));
         // x1
         fl += (R)d[1][0] * (R)M(1, 0, 0) * (U(1, 0, 0) - uc);
         fl += minor_flux<R>((R)d[1][1], M(1, -1, 0), U(1, -1, 0), M(0, -1, 0), U(0, -1, 0),
-                            M(1, 0, 0), U(1, 0, 0), 1, uc, M(1, 1, 0), U(1, 1, 0),
-                            M(0, 1, 0), U(0, 1, 0));
+                            M(1, 0, 0), U(1, 0, 0), 1, uc,
+                            M(1, 1, 0), U(1, 1, 0), M(0, 1, 0), U(0, 1, 0));
         fl += minor_flux<R>((R)d[1][2], M(1, 0, -1), U(1, 0, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(1, 0, 0), U(1, 0, 0), 1, uc, M(1, 0, 1), U(1, 0, 1),
-                            M(0, 0, 1), U(0, 0, 1));
+                            M(1, 0, 0), U(1, 0, 0), 1, uc,
+                            M(1, 0, 1), U(1, 0, 1), M(0, 0, 1), U(0, 0, 1));
         // y0
         fl += (R)d[2][0] * (R)M(0, -1, 0) * (U(0, -1, 0) - uc);
         fl -= minor_flux<R>((R)d[2][1], M(-1, -1, 0), U(-1, -1, 0), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, -1, 0), U(0, -1, 0), 1, uc, M(1, -1, 0), U(1, -1, 0),
-                            M(1, 0, 0), U(1, 0, 0));
+                            M(0, -1, 0), U(0, -1, 0), 1, uc,
+                            M(1, -1, 0), U(1, -1, 0), M(1, 0, 0), U(1, 0, 0));
         fl -= minor_flux<R>((R)d[2][2], M(0, -1, -1), U(0, -1, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(0, -1, 0), U(0, -1, 0), 1, uc, M(0, -1, 1), U(0, -1, 1),
-                            M(0, 0, 1), U(0, 0, 1));
+                            M(0, -1, 0), U(0, -1, 0), 1, uc,
+                            M(0, -1, 1), U(0, -1, 1), M(0, 0, 1), U(0, 0, 1));
         // y1
         fl += (R)d[3][0] * (R)M(0, 1, 0) * (U(0, 1, 0) - uc);
         fl += minor_flux<R>((R)d[3][1], M(-1, 1, 0), U(-1, 1, 0), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, 1, 0), U(0, 1, 0), 1, uc, M(1, 1, 0), U(1, 1, 0),
-                            M(1, 0, 0), U(1, 0, 0));
+                            M(0, 1, 0), U(0, 1, 0), 1, uc,
+                            M(1, 1, 0), U(1, 1, 0), M(1, 0, 0), U(1, 0, 0));
         fl += minor_flux<R>((R)d[3][2], M(0, 1, -1), U(0, 1, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(0, 1, 0), U(0, 1, 0), 1, uc, M(0, 1, 1), U(0, 1, 1),
-                            M(0, 0, 1), U(0, 0, 1));
+                            M(0, 1, 0), U(0, 1, 0), 1, uc,
+                            M(0, 1, 1), U(0, 1, 1), M(0, 0, 1), U(0, 0, 1));
         // z0
         fl += (R)d[4][0] * (R)M(0, 0, -1) * (U(0, 0, -1) - uc);
         fl -= minor_flux<R>((R)d[4][1], M(-1, 0, -1), U(-1, 0, -1), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, 0, -1), U(0, 0, -1), 1, uc, M(1, 0, -1), U(1, 0, -1),
-                            M(1, 0, 0), U(1, 0, 0));
+                            M(0, 0, -1), U(0, 0, -1), 1, uc,
+                            M(1, 0, -1), U(1, 0, -1), M(1, 0, 0), U(1, 0, 0));
         fl -= minor_flux<R>((R)d[4][2], M(0, -1, -1), U(0, -1, -1), M(0, -1, 0), U(0, -1, 0),
-                            M(0, 0, -1), U(0, 0, -1), 1, uc, M(0, 1, -1), U(0, 1, -1),
-                            M(0, 1, 0), U(0, 1, 0));
+                            M(0, 0, -1), U(0, 0, -1), 1, uc,
+                            M(0, 1, -1), U(0, 1, -1), M(0, 1, 0), U(0, 1, 0));
         // z1
         fl += (R)d[5][0] * (R)M(0, 0, 1) * (U(0, 0, 1) - uc);
         fl += minor_flux<R>((R)d[5][1], M(-1, 0, 1), U(-1, 0, 1), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, 0, 1), U(0, 0, 1), 1, uc, M(1, 0, 1), U(1, 0, 1),
-                            M(1, 0, 0), U(1, 0, 0));
+                            M(0, 0, 1), U(0, 0, 1), 1, uc,
+                            M(1, 0, 1), U(1, 0, 1), M(1, 0, 0), U(1, 0, 0));
         fl += minor_flux<R>((R)d[5][2], M(0, -1, 1), U(0, -1, 1), M(0, -1, 0), U(0, -1, 0),
-                            M(0, 0, 1), U(0, 0, 1), 1, uc, M(0, 1, 1), U(0, 1, 1),
-                            M(0, 1, 0), U(0, 1, 0));
+                            M(0, 0, 1), U(0, 0, 1), 1, uc,
+                            M(0, 1, 1), U(0, 1, 1), M(0, 1, 0), U(0, 1, 0));
         return uc + a.s * fl;
       }
     }
@@ -611,16 +719,21 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
 }
 
 // Waves per SIMD the compiler must leave room for.  The 2D FP64 TP06 step
-// would take 136-156 registers (3 waves); a 128-register budget fits 4 at the
-// cost of 6 spilled registers and is 2.6 % faster; 96 registers (5 waves)
-// spill 36 and are 9 % slower.  The 3D anisotropic one would spill 74 under
-// the same budget.  Other steps keep the compiler's choice.
+// takes 110 registers under a 128-register budget (4 waves, no spills); left
+// to itself the compiler would choose 3 waves.  The 2D FP32 TP06 step with
+// its neighbourhood loaded early takes 105 registers (4 waves) unless held
+// to 96 (5 waves).  The 3D anisotropic FP64 step would spill 74 registers
+// under a 128-register budget.  Other steps keep the compiler's choice.
 template <int NDIM, int MODEL, int PREC>
 constexpr int step_min_waves() {
-  return (NDIM == 2 && MODEL == 6 && PREC == 0) ? 4 : 1;
+  return (NDIM == 2 && MODEL == 6) ? (PREC == 0 ? 4 : 5) : 1;
 }
 
-template <int NDIM, int ST, int MODEL, typename R, int PREC>
+// UNI (2D FP32 flux form only): every field is uniform, launched when the
+// plan's fields are (KArgs::uni_fields); the flux then reads the field values
+// without the per-field mode dispatch, whose dependent argument loads
+// stalled the waves (1026^2 TP06 FP32: 42 -> 40 us per step)
+template <int NDIM, int ST, int MODEL, typename R, int PREC, bool UNI>
 __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void k_step(
     const KArgs<R, UType<MODEL, R>> a) {
   using U = UType<MODEL, R>;
@@ -642,9 +755,18 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void
     bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
     if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
     if (interior) {
-      a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
-        return diffuse<NDIM, ST, R, U, PREC == 0>(a, i, j, k, c, uc);
-      });
+      if constexpr (NDIM == 2 && PREC != 0 && ST != 2) {
+        // FP32 flux form: the neighbourhood is loaded before the ionic
+        // update, as in the batched step
+        const Hood2<U> q = load_hood2<ST>(a, c, uc);
+        a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
+          return flux2d<ST, UNI>(a, i, j, c, uc, q);
+        });
+      } else {
+        a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
+          return diffuse<NDIM, ST, R, U, PREC == 0>(a, i, j, k, c, uc);
+        });
+      }
     }
   }
   if ((code & 4) && a.probe_row >= 0) {

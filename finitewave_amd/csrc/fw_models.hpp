@@ -985,8 +985,9 @@ __host__ __device__ constexpr int model_n_state(int model) {
 // neighbour values and the diffusion result do not stay live across the
 // ionic code; the light models keep the literal order, which hides the
 // neighbour loads better.  Without contraction (FP64) both orders give the
-// reference's value exactly; with contraction (FP32) the single-tissue and
-// batched kernels both use this function, so they stay bit-identical.
+// reference's value exactly.  FP32 builds fuse multiply-adds within one
+// expression only, so the single-tissue and batched kernels, which both use
+// this function, stay bit-identical.
 template <int MODEL, typename R, typename U, typename SP, typename Diffusion>
 __device__ __forceinline__ U node_update(U uc, SP& S, int64_t c, R dt, const Params& P,
                                          Diffusion&& diffusion) {

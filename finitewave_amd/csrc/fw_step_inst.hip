@@ -1,7 +1,7 @@
 // fw_step_inst.hip -- instantiates the fused step kernel for one
 // (precision, model) pair, chosen at compile time by FW_PREC and FW_MODEL.
 // The build compiles this file once per pair (FP64 builds with
-// -ffp-contract=off, FP32 builds with contraction allowed).
+// -ffp-contract=off, FP32 builds with -ffp-contract=on).
 #include <type_traits>
 
 #include "fw_internal.hpp"
@@ -25,7 +25,14 @@ using UReal = UType<FW_MODEL, Real>;  // storage type of u / u_new
 
 template <int NDIM, int ST>
 static void launch(const fw_plan* plan, const KArgs<Real, UReal>& a, hipStream_t s) {
-  hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC>), dim3(plan->grid),
+  if constexpr (NDIM == 2 && FW_PREC != 0 && ST != FW_STENCIL_WEIGHTS) {
+    if (a.uni_fields) {
+      hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, true>), dim3(plan->grid),
+                         dim3(kBlock), 0, s, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, false>), dim3(plan->grid),
                      dim3(kBlock), 0, s, a);
 }
 
@@ -57,6 +64,9 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
       a.f[q] = Field{0, 0, 0.0, nullptr, 0.0f, nullptr};
     }
   }
+  a.uni_fields = 1;
+  for (int q = 0; q < d.n_fields; ++q)
+    if (d.fields[q].mode != 0) a.uni_fields = 0;
   if (d.stencil == FW_STENCIL_WEIGHTS) a.f[0].mode = plan->K;
   a.w = d.weights;
   a.D = d.D_model;

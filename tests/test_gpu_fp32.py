@@ -205,3 +205,45 @@ def test_fp32_one_step_with_tabulated_fields_matches_fp64_oracle(gpu, case):
     sim.u, sim.u_new = sim.u_new, sim.u
     act = model.cardiac_tissue.mesh == 1
     assert float(np.max(np.abs(u_gpu[act] - sim.u[act]))) <= 1e-5
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,aniso", [("TP062D", True), ("AlievPanfilov2D", True),
+                                        ("AlievPanfilov2D", False)])
+def test_uniform_field_kernel_equals_general_kernel(gpu, name, aniso):
+    """The step kernel instantiated for uniform fields and the general one
+    give the same bits.  The second tissue's conductivity differs at the
+    corner node (0, 0) only: no updated node's half-step fields read it, so
+    the potentials are the same, but its plan is no longer uniform and the
+    general kernel runs, reading its fields from the plan's tables."""
+    from finitewave_amd.stencil import compress_field
+    shape = (48, 40)
+    runs = []
+    for corner in (1.0, 0.5):
+        tissue = fw.CardiacTissue2D(list(shape))
+        if aniso:
+            tissue.fibers = _fibres_2d(np.full((1, 1), np.deg2rad(30.0)), shape)
+        cond = np.ones(shape)
+        cond[0, 0] = corner
+        tissue.conductivity = cond
+        model = getattr(fw, name)()
+        model.dt, model.dr, model.t_max = 0.01, 0.25, 3.0
+        model.precision = "fp32"
+        model.prog_bar = False
+        model.cardiac_tissue = tissue
+        seq = fw.StimSequence()
+        seq.add_stim(fw.StimCurrentCoord2D(0, 100 if name == "TP062D" else 5, 0.5,
+                                           0, shape[0], 0, 5))
+        model.stim_sequence = seq
+        model.initialize()
+        modes = {compress_field(a)[0]
+                 for a in model.stencil.half_step_fields(model, tissue)}
+        assert (modes == {0}) == (corner == 1.0), modes  # uniform-field kernel or general
+        model.run(initialize=False)
+        runs.append((model.u.copy(), {n: getattr(model, n).copy()
+                                      for n in model._state_call_order()}))
+    (u_a, st_a), (u_b, st_b) = runs
+    assert np.any(u_a != u_a.flat[0])
+    assert np.array_equal(u_a, u_b)
+    for n in st_a:
+        assert np.array_equal(st_a[n], st_b[n]), n
