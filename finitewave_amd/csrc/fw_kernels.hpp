@@ -498,8 +498,9 @@ __device__ __forceinline__ R flux2d(const KArgs<RA, R>& a, int64_t i, int64_t j,
 
 // R here is the potential's storage type (FP32 path of the millivolt-scale
 // models: double); EXACT selects the reference's FP64 weights, else the flux
-// form evaluated in R
-template <int NDIM, int ST, typename RA, typename R, bool EXACT>
+// form evaluated in R; UNI (2D): every field is uniform, read without the
+// per-field mode dispatch
+template <int NDIM, int ST, typename RA, typename R, bool EXACT, bool UNI = false>
 __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j,
                                      int64_t k, int64_t c, R uc) {
   const int64_t si = a.nj * a.nk, sj = a.nk;  // 2D: nk = 1, sj = 1, si = nj
@@ -522,14 +523,20 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
                  (double)u[c + sN + 1] * W[8]);
     } else if constexpr (!EXACT) {
       const Hood2<R> q = load_hood2<ST>(a, c, uc);
+      if constexpr (UNI) return flux2d<ST, true>(a, i, j, c, uc, q);
       return a.uni_fields ? flux2d<ST, true>(a, i, j, c, uc, q)
                           : flux2d<ST, false>(a, i, j, c, uc, q);
     } else if constexpr (ST == 0) {
       const int mN = a.code[c - sN] & 1, mS = a.code[c + sN] & 1;
       const int mW = a.code[c - 1] & 1, mE = a.code[c + 1] & 1;
       R dN, dS, dW, dE;
-      fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dS, dN);
-      fval_pair<R, 1>(a.f[1], i, j, 0, c, 1, dE, dW);
+      if constexpr (UNI) {
+        dN = dS = funi<R>(a.f[0]);
+        dW = dE = funi<R>(a.f[1]);
+      } else {
+        fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dS, dN);
+        fval_pair<R, 1>(a.f[1], i, j, 0, c, 1, dE, dW);
+      }
       double W[5];
       W[0] = iso_comp(dN, mN, mS);
       W[1] = iso_comp(dW, mW, mE);
@@ -550,10 +557,17 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
           M.m[(di + 1) * 9 + (dj + 1) * 3 + 1] = a.code[c + di * sN + dj] & 1;
         }
       R dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1;
-      fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dxx1, dxx0);
-      fval_pair<R, 0>(a.f[1], i, j, 0, c, sN, dxy1, dxy0);
-      fval_pair<R, 1>(a.f[2], i, j, 0, c, 1, dyx1, dyx0);
-      fval_pair<R, 1>(a.f[3], i, j, 0, c, 1, dyy1, dyy0);
+      if constexpr (UNI) {
+        dxx0 = dxx1 = funi<R>(a.f[0]);
+        dxy0 = dxy1 = funi<R>(a.f[1]);
+        dyx0 = dyx1 = funi<R>(a.f[2]);
+        dyy0 = dyy1 = funi<R>(a.f[3]);
+      } else {
+        fval_pair<R, 0>(a.f[0], i, j, 0, c, sN, dxx1, dxx0);
+        fval_pair<R, 0>(a.f[1], i, j, 0, c, sN, dxy1, dxy0);
+        fval_pair<R, 1>(a.f[2], i, j, 0, c, 1, dyx1, dyx0);
+        fval_pair<R, 1>(a.f[3], i, j, 0, c, 1, dyy1, dyy0);
+      }
       double W[9];
       wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
       scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
@@ -729,9 +743,9 @@ constexpr int step_min_waves() {
   return (NDIM == 2 && MODEL == 6) ? (PREC == 0 ? 4 : 5) : 1;
 }
 
-// UNI (2D FP32 flux form only): every field is uniform, launched when the
-// plan's fields are (KArgs::uni_fields); the flux then reads the field values
-// without the per-field mode dispatch, whose dependent argument loads
+// UNI (2D, computed stencils): every field is uniform, launched when the
+// plan's fields are (KArgs::uni_fields); the stencil then reads the field
+// values without the per-field mode dispatch, whose dependent argument loads
 // stalled the waves (1026^2 TP06 FP32: 42 -> 40 us per step)
 template <int NDIM, int ST, int MODEL, typename R, int PREC, bool UNI>
 __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void k_step(
@@ -764,7 +778,7 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void
         });
       } else {
         a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
-          return diffuse<NDIM, ST, R, U, PREC == 0>(a, i, j, k, c, uc);
+          return diffuse<NDIM, ST, R, U, PREC == 0, UNI>(a, i, j, k, c, uc);
         });
       }
     }

@@ -25,7 +25,7 @@ using UReal = UType<FW_MODEL, Real>;  // storage type of u / u_new
 
 template <int NDIM, int ST>
 static void launch(const fw_plan* plan, const KArgs<Real, UReal>& a, hipStream_t s) {
-  if constexpr (NDIM == 2 && FW_PREC != 0 && ST != FW_STENCIL_WEIGHTS) {
+  if constexpr (NDIM == 2 && ST != FW_STENCIL_WEIGHTS) {
     if (a.uni_fields) {
       hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, true>), dim3(plan->grid),
                          dim3(kBlock), 0, s, a);
