@@ -85,6 +85,17 @@ __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ d
   if (q < n) dst[q] = (float)src[q];
 }
 
+// A field as a table along the last axis (n entries): a copy of its table,
+// or its uniform value repeated (src null); in double and in float.
+__global__ void k_fill_ktab(const double* __restrict__ src, double value, double* __restrict__ kd,
+                            float* __restrict__ kf, int64_t n) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double v = src ? src[q] : value;
+  kd[q] = v;
+  kf[q] = (float)v;
+}
+
 // Admission check of a batched tissue's codes: bit 0 of *bad when an updated
 // node (code & 3 == 1) lies on the outer layer, bit 1 when a node is frozen
 // (code bit 1).  Grid-stride over all n = ni * nj nodes.
@@ -319,6 +330,43 @@ int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
       }
     }
   }
+  // the field layout: every field uniform (kFieldsUniform), or in 3D every
+  // field uniform or tabulated along the last axis (kFieldsK, layered
+  // fibres); such plans get their step kernel without the per-field mode
+  // dispatch
+  p->field_layout = fw::kFieldsGeneral;
+  if (d.stencil != FW_STENCIL_WEIGHTS) {
+    bool uni = true, along_k = d.ndim == 3;
+    for (int q = 0; q < need_fields; ++q) {
+      const fw_field& f = d.fields[q];
+      if (f.mode != FW_FIELD_UNIFORM) uni = false;
+      if (!(f.mode == FW_FIELD_UNIFORM || (f.mode == FW_FIELD_AXIS && f.axis == 2)))
+        along_k = false;
+    }
+    if (uni) {
+      p->field_layout = fw::kFieldsUniform;
+    } else if (along_k) {
+      const int64_t nk = d.shape[2];
+      (void)hipDeviceSynchronize();  // the tables' uploads on any stream
+      for (int q = 0; q < need_fields && e == hipSuccess; ++q) {
+        const fw_field& f = d.fields[q];
+        e = hipMalloc(&p->ktab[q], nk * sizeof(double));
+        if (e == hipSuccess) e = hipMalloc(&p->fktab[q], nk * sizeof(float));
+        if (e == hipSuccess)
+          hipLaunchKernelGGL(k_fill_ktab, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, 0,
+                             f.mode == FW_FIELD_AXIS ? f.data : nullptr, f.value, p->ktab[q],
+                             p->fktab[q], nk);
+        if (e == hipSuccess) e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      if (e != hipSuccess) {
+        set_error("fw_plan_create: field tables along k: %s", hipGetErrorString(e));
+        fw_plan_destroy(p);
+        return FW_E_NOMEM;
+      }
+      p->field_layout = fw::kFieldsK;
+    }
+  }
   *out = p;
   return FW_OK;
 }
@@ -340,8 +388,11 @@ int fw_plan_destroy(fw_plan* plan) {
   if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
   if (plan->graph_long) (void)hipGraphExecDestroy(plan->graph_long);
   if (plan->step_words) (void)hipFree(plan->step_words);
-  for (int q = 0; q < FW_MAX_FIELDS; ++q)
+  for (int q = 0; q < FW_MAX_FIELDS; ++q) {
     if (plan->ftab[q]) (void)hipFree(plan->ftab[q]);
+    if (plan->ktab[q]) (void)hipFree(plan->ktab[q]);
+    if (plan->fktab[q]) (void)hipFree(plan->fktab[q]);
+  }
   delete plan;
   return FW_OK;
 }

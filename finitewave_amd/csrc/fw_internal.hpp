@@ -26,6 +26,12 @@ struct fw_plan {
   // FP32 plans with a float potential: float copies of the tabulated
   // half-step fields (made once by fw_plan_create)
   float* ftab[FW_MAX_FIELDS];
+  // the field layout the step kernel is instantiated for (fw::FieldLayout);
+  // kFieldsK plans hold every field as a table along the last axis, uniform
+  // ones filled with their value, in double and in float
+  int32_t field_layout;
+  double* ktab[FW_MAX_FIELDS];
+  float* fktab[FW_MAX_FIELDS];
 };
 
 namespace fw {

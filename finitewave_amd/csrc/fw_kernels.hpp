@@ -36,6 +36,12 @@ struct Field {
   const float* fdata;  // the plan's float copy of data
 };
 
+// How a plan's half-step fields are stored, as far as the step kernel is
+// specialised for it: kFieldsUniform, every field uniform; kFieldsK (3D),
+// every field a table along the last axis (the plan tabulates uniform ones
+// too: layered fibres); kFieldsGeneral, a mode per field, read by dispatch.
+enum FieldLayout : int32_t { kFieldsGeneral = 0, kFieldsUniform = 1, kFieldsK = 2 };
+
 // a half-step field at node (i, j, k) = flat index c, in the type F the
 // diffusion is computed in; the mode is uniform across a launch
 template <typename F>
@@ -66,6 +72,25 @@ __device__ __forceinline__ void fval_pair(const Field& f, int64_t i, int64_t j, 
     below = at_c;
   else
     below = fval<F>(f, i - (DIR == 0), j - (DIR == 1), k - (DIR == 2), c - step);
+}
+
+// fval_pair for a kernel specialised to a field layout FL: uniform values
+// straight from the arguments, tables along the last axis indexed by k, or
+// the general per-field dispatch
+template <typename F, int DIR, int FL>
+__device__ __forceinline__ void fval_pair_l(const Field& f, int64_t i, int64_t j, int64_t k,
+                                            int64_t c, int64_t step, F& at_c, F& below) {
+  if constexpr (FL == kFieldsUniform) {
+    at_c = below = funi<F>(f);
+  } else if constexpr (FL == kFieldsK) {
+    const global_t<F>* t;
+    if constexpr (sizeof(F) == 4) t = (const global_t<F>*)f.fdata;
+    else t = (const global_t<F>*)f.data;
+    at_c = t[k];
+    below = DIR == 2 ? t[k - 1] : at_c;
+  } else {
+    fval_pair<F, DIR>(f, i, j, k, c, step, at_c, below);
+  }
 }
 
 // Division of 32-bit unsigned numerators by a divisor d >= 1 fixed for a
@@ -119,7 +144,7 @@ struct KArgs {
   int32_t probe_row;
   int32_t idx32;      // N < 2^32: node coordinates by 32-bit division
   UDiv div_jk, div_k;  // division by nj * nk and by nk
-  int32_t uni_fields;  // every field uniform (mode 0)
+  int32_t field_layout;  // FieldLayout of the plan's fields
 };
 
 // (i, j, k) of flat node index c in a C-order grid (k = 0 in 2D)
@@ -498,9 +523,8 @@ __device__ __forceinline__ R flux2d(const KArgs<RA, R>& a, int64_t i, int64_t j,
 
 // R here is the potential's storage type (FP32 path of the millivolt-scale
 // models: double); EXACT selects the reference's FP64 weights, else the flux
-// form evaluated in R; UNI (2D): every field is uniform, read without the
-// per-field mode dispatch
-template <int NDIM, int ST, typename RA, typename R, bool EXACT, bool UNI = false>
+// form evaluated in R; FL: the field layout the caller is specialised for
+template <int NDIM, int ST, typename RA, typename R, bool EXACT, int FL = kFieldsGeneral>
 __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j,
                                      int64_t k, int64_t c, R uc) {
   const int64_t si = a.nj * a.nk, sj = a.nk;  // 2D: nk = 1, sj = 1, si = nj
@@ -523,14 +547,14 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
                  (double)u[c + sN + 1] * W[8]);
     } else if constexpr (!EXACT) {
       const Hood2<R> q = load_hood2<ST>(a, c, uc);
-      if constexpr (UNI) return flux2d<ST, true>(a, i, j, c, uc, q);
-      return a.uni_fields ? flux2d<ST, true>(a, i, j, c, uc, q)
-                          : flux2d<ST, false>(a, i, j, c, uc, q);
+      if constexpr (FL == kFieldsUniform) return flux2d<ST, true>(a, i, j, c, uc, q);
+      return a.field_layout == kFieldsUniform ? flux2d<ST, true>(a, i, j, c, uc, q)
+                                              : flux2d<ST, false>(a, i, j, c, uc, q);
     } else if constexpr (ST == 0) {
       const int mN = a.code[c - sN] & 1, mS = a.code[c + sN] & 1;
       const int mW = a.code[c - 1] & 1, mE = a.code[c + 1] & 1;
       R dN, dS, dW, dE;
-      if constexpr (UNI) {
+      if constexpr (FL == kFieldsUniform) {
         dN = dS = funi<R>(a.f[0]);
         dW = dE = funi<R>(a.f[1]);
       } else {
@@ -557,7 +581,7 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
           M.m[(di + 1) * 9 + (dj + 1) * 3 + 1] = a.code[c + di * sN + dj] & 1;
         }
       R dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1;
-      if constexpr (UNI) {
+      if constexpr (FL == kFieldsUniform) {
         dxx0 = dxx1 = funi<R>(a.f[0]);
         dxy0 = dxy1 = funi<R>(a.f[1]);
         dyx0 = dyx1 = funi<R>(a.f[2]);
@@ -599,9 +623,9 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
       const int mYm = a.code[c - sj] & 1, mYp = a.code[c + sj] & 1;
       const int mZm = a.code[c - 1] & 1, mZp = a.code[c + 1] & 1;
       R dXm, dXp, dYm, dYp, dZm, dZp;
-      fval_pair<R, 0>(a.f[0], i, j, k, c, si, dXp, dXm);
-      fval_pair<R, 1>(a.f[1], i, j, k, c, sj, dYp, dYm);
-      fval_pair<R, 2>(a.f[2], i, j, k, c, 1, dZp, dZm);
+      fval_pair_l<R, 0, FL>(a.f[0], i, j, k, c, si, dXp, dXm);
+      fval_pair_l<R, 1, FL>(a.f[1], i, j, k, c, sj, dYp, dYm);
+      fval_pair_l<R, 2, FL>(a.f[2], i, j, k, c, 1, dZp, dZm);
       if constexpr (EXACT) {
         double W[7];
         W[0] = iso_comp(dXm, mXm, mXp);
@@ -640,15 +664,15 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
           }
       // fields: 0 xx 1 xy 2 xz 3 yx 4 yy 5 yz 6 zx 7 zy 8 zz
       R d[6][3];
-      fval_pair<R, 0>(a.f[0], i, j, k, c, si, d[1][0], d[0][0]);
-      fval_pair<R, 0>(a.f[1], i, j, k, c, si, d[1][1], d[0][1]);
-      fval_pair<R, 0>(a.f[2], i, j, k, c, si, d[1][2], d[0][2]);
-      fval_pair<R, 1>(a.f[4], i, j, k, c, sj, d[3][0], d[2][0]);
-      fval_pair<R, 1>(a.f[3], i, j, k, c, sj, d[3][1], d[2][1]);
-      fval_pair<R, 1>(a.f[5], i, j, k, c, sj, d[3][2], d[2][2]);
-      fval_pair<R, 2>(a.f[8], i, j, k, c, 1, d[5][0], d[4][0]);
-      fval_pair<R, 2>(a.f[6], i, j, k, c, 1, d[5][1], d[4][1]);
-      fval_pair<R, 2>(a.f[7], i, j, k, c, 1, d[5][2], d[4][2]);
+      fval_pair_l<R, 0, FL>(a.f[0], i, j, k, c, si, d[1][0], d[0][0]);
+      fval_pair_l<R, 0, FL>(a.f[1], i, j, k, c, si, d[1][1], d[0][1]);
+      fval_pair_l<R, 0, FL>(a.f[2], i, j, k, c, si, d[1][2], d[0][2]);
+      fval_pair_l<R, 1, FL>(a.f[4], i, j, k, c, sj, d[3][0], d[2][0]);
+      fval_pair_l<R, 1, FL>(a.f[3], i, j, k, c, sj, d[3][1], d[2][1]);
+      fval_pair_l<R, 1, FL>(a.f[5], i, j, k, c, sj, d[3][2], d[2][2]);
+      fval_pair_l<R, 2, FL>(a.f[8], i, j, k, c, 1, d[5][0], d[4][0]);
+      fval_pair_l<R, 2, FL>(a.f[6], i, j, k, c, 1, d[5][1], d[4][1]);
+      fval_pair_l<R, 2, FL>(a.f[7], i, j, k, c, 1, d[5][2], d[4][2]);
       if constexpr (EXACT) {
         double W[19];
         wts_aniso3d(W, M, d);
@@ -743,11 +767,11 @@ constexpr int step_min_waves() {
   return (NDIM == 2 && MODEL == 6) ? (PREC == 0 ? 4 : 5) : 1;
 }
 
-// UNI (2D, computed stencils): every field is uniform, launched when the
-// plan's fields are (KArgs::uni_fields); the stencil then reads the field
-// values without the per-field mode dispatch, whose dependent argument loads
-// stalled the waves (1026^2 TP06 FP32: 42 -> 40 us per step)
-template <int NDIM, int ST, int MODEL, typename R, int PREC, bool UNI>
+// FL: the plan's field layout (KArgs::field_layout).  The uniform and
+// along-k instances read the field values without the per-field mode
+// dispatch, whose dependent argument loads stalled the waves (1026^2 TP06
+// FP32: 42 -> 40 us per step)
+template <int NDIM, int ST, int MODEL, typename R, int PREC, int FL>
 __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void k_step(
     const KArgs<R, UType<MODEL, R>> a) {
   using U = UType<MODEL, R>;
@@ -774,11 +798,11 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void
         // update, as in the batched step
         const Hood2<U> q = load_hood2<ST>(a, c, uc);
         a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
-          return flux2d<ST, UNI>(a, i, j, c, uc, q);
+          return flux2d<ST, FL == kFieldsUniform>(a, i, j, c, uc, q);
         });
       } else {
         a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
-          return diffuse<NDIM, ST, R, U, PREC == 0, UNI>(a, i, j, k, c, uc);
+          return diffuse<NDIM, ST, R, U, PREC == 0, FL>(a, i, j, k, c, uc);
         });
       }
     }

@@ -25,15 +25,22 @@ using UReal = UType<FW_MODEL, Real>;  // storage type of u / u_new
 
 template <int NDIM, int ST>
 static void launch(const fw_plan* plan, const KArgs<Real, UReal>& a, hipStream_t s) {
-  if constexpr (NDIM == 2 && ST != FW_STENCIL_WEIGHTS) {
-    if (a.uni_fields) {
-      hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, true>), dim3(plan->grid),
-                         dim3(kBlock), 0, s, a);
+  if constexpr (ST != FW_STENCIL_WEIGHTS) {
+    if (a.field_layout == kFieldsUniform) {
+      hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, kFieldsUniform>),
+                         dim3(plan->grid), dim3(kBlock), 0, s, a);
       return;
     }
+    if constexpr (NDIM == 3) {
+      if (a.field_layout == kFieldsK) {
+        hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, kFieldsK>),
+                           dim3(plan->grid), dim3(kBlock), 0, s, a);
+        return;
+      }
+    }
   }
-  hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, false>), dim3(plan->grid),
-                     dim3(kBlock), 0, s, a);
+  hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, kFieldsGeneral>),
+                     dim3(plan->grid), dim3(kBlock), 0, s, a);
 }
 
 static void fill_args(const fw_plan* plan, const void* u, void* u_new, const StatePtrs& st,
@@ -64,9 +71,17 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
       a.f[q] = Field{0, 0, 0.0, nullptr, 0.0f, nullptr};
     }
   }
-  a.uni_fields = 1;
-  for (int q = 0; q < d.n_fields; ++q)
-    if (d.fields[q].mode != 0) a.uni_fields = 0;
+  a.field_layout = plan->field_layout;
+  if (plan->field_layout == kFieldsK) {
+    // every field as its table along the last axis, which the general
+    // dispatch (the ECG kernel) reads the same way
+    for (int q = 0; q < d.n_fields; ++q) {
+      a.f[q].mode = 1;
+      a.f[q].axis = 2;
+      a.f[q].data = plan->ktab[q];
+      a.f[q].fdata = plan->fktab[q];
+    }
+  }
   if (d.stencil == FW_STENCIL_WEIGHTS) a.f[0].mode = plan->K;
   a.w = d.weights;
   a.D = d.D_model;

@@ -208,37 +208,50 @@ def test_fp32_one_step_with_tabulated_fields_matches_fp64_oracle(gpu, case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name,aniso", [("TP062D", True), ("AlievPanfilov2D", True),
-                                        ("AlievPanfilov2D", False)])
-def test_uniform_field_kernel_equals_general_kernel(gpu, name, aniso):
-    """The step kernel instantiated for uniform fields and the general one
-    give the same bits.  The second tissue's conductivity differs at the
-    corner node (0, 0) only: no updated node's half-step fields read it, so
-    the potentials are the same, but its plan is no longer uniform and the
-    general kernel runs, reading its fields from the plan's tables."""
+@pytest.mark.parametrize("name,fibres,precision", [
+    ("TP062D", "uniform", "fp32"), ("AlievPanfilov2D", "uniform", "fp32"),
+    ("AlievPanfilov2D", None, "fp32"), ("TP062D", "uniform", "fp64"),
+    ("AlievPanfilov3D", "uniform", "fp32"), ("AlievPanfilov3D", "layered", "fp32"),
+    ("AlievPanfilov3D", "layered", "fp64"), ("AlievPanfilov3D", None, "fp64")])
+def test_field_layout_kernels_equal_general_kernel(gpu, name, fibres, precision):
+    """The step kernels instantiated for a field layout (every field uniform;
+    in 3D, every field uniform or a table along k, as with layered fibres)
+    give the general kernel's bits.  The second tissue's conductivity
+    differs at the corner node (0, 0(, 0)) only: no updated node's half-step
+    fields read it, so the potentials are the same, but its plan falls back
+    to the general kernel, which reads its fields by per-field dispatch."""
     from finitewave_amd.stencil import compress_field
-    shape = (48, 40)
+    dim = 3 if name.endswith("3D") else 2
+    shape = (48, 40) if dim == 2 else (14, 12, 16)
     runs = []
     for corner in (1.0, 0.5):
-        tissue = fw.CardiacTissue2D(list(shape))
-        if aniso:
-            tissue.fibers = _fibres_2d(np.full((1, 1), np.deg2rad(30.0)), shape)
+        tissue = (fw.CardiacTissue2D if dim == 2 else fw.CardiacTissue3D)(list(shape))
+        if fibres == "uniform":
+            f = np.zeros((*shape, dim))
+            f[..., 0], f[..., 1] = np.cos(np.deg2rad(30.0)), np.sin(np.deg2rad(30.0))
+            tissue.fibers = f
+        elif fibres == "layered":
+            tissue.fibers = layered_fibers(shape)
         cond = np.ones(shape)
-        cond[0, 0] = corner
+        cond[(0,) * dim] = corner
         tissue.conductivity = cond
         model = getattr(fw, name)()
         model.dt, model.dr, model.t_max = 0.01, 0.25, 3.0
-        model.precision = "fp32"
+        model.precision = precision
         model.prog_bar = False
         model.cardiac_tissue = tissue
         seq = fw.StimSequence()
-        seq.add_stim(fw.StimCurrentCoord2D(0, 100 if name == "TP062D" else 5, 0.5,
-                                           0, shape[0], 0, 5))
+        box = (0, shape[0], 0, 5) + ((0, shape[2]) if dim == 3 else ())
+        stim = fw.StimCurrentCoord2D if dim == 2 else fw.StimCurrentCoord3D
+        seq.add_stim(stim(0, 100 if name.startswith("TP06") else 5, 0.5, *box))
         model.stim_sequence = seq
         model.initialize()
         modes = {compress_field(a)[0]
                  for a in model.stencil.half_step_fields(model, tissue)}
-        assert (modes == {0}) == (corner == 1.0), modes  # uniform-field kernel or general
+        if corner == 1.0:   # a specialised kernel runs
+            assert modes <= ({0} if fibres != "layered" else {0, 1}), modes
+        else:               # the general one
+            assert 2 in modes, modes
         model.run(initialize=False)
         runs.append((model.u.copy(), {n: getattr(model, n).copy()
                                       for n in model._state_call_order()}))
