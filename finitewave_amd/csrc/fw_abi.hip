@@ -85,6 +85,22 @@ __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ d
   if (q < n) dst[q] = (float)src[q];
 }
 
+// The quotients minor_exact forms from a field value d and the counts
+// mh, ml in {3, 4}, in its operation order: -d / ml, d / mh and
+// d * (1 / mh - 1 / ml).  This unit is built without contraction.
+void minor_quotients(double d, MinorQ& T) {
+  for (int h = 0; h < 2; ++h) {
+    const double dm = h ? 4.0 : 3.0;
+    T.nq[h] = (-d) / dm;
+    T.pq[h] = d / dm;
+    for (int l = 0; l < 2; ++l) {
+      const double dl = l ? 4.0 : 3.0;
+      const double c = 1.0 / dm - 1.0 / dl;
+      T.dc[h][l] = d * c;
+    }
+  }
+}
+
 // A field as a table along the last axis (n entries): a copy of its table,
 // or its uniform value repeated (src null); in double and in float.
 __global__ void k_fill_ktab(const double* __restrict__ src, double value, double* __restrict__ kd,

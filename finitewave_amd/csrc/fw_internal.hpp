@@ -36,6 +36,11 @@ struct fw_plan {
 
 namespace fw {
 
+struct MinorQ;
+// d's cross-term quotients for the uniform-field FP64 step kernels, computed
+// on the host without contraction (fw_abi.hip)
+void minor_quotients(double d, MinorQ& T);
+
 struct TrackArgs {
   int32_t* act;
   double act_thr;

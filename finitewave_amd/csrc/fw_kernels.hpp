@@ -114,6 +114,15 @@ __device__ __forceinline__ uint32_t udiv(uint32_t n, UDiv v) {
   return (t + ((n - t) >> v.s1)) >> v.s2;
 }
 
+// The quotients of a cross term of the exact weights for one uniform field
+// value d (the counts mh, ml are 3 or 4 whenever the term is on), index
+// [count == 4]: nq = -d/ml, pq = d/mh, dc = d * (1/mh - 1/ml).  The host
+// fills them with the same IEEE operations (minor_quotients), so a kernel
+// for uniform fields selects instead of dividing, with the same bits.
+struct MinorQ {
+  double nq[2], pq[2], dc[2][2];
+};
+
 // R: arithmetic type of the ionic update; U: storage type of u / u_new
 // (UType<MODEL, R>, double for the millivolt-scale models on the FP32 path)
 template <typename R, typename U = R>
@@ -145,6 +154,7 @@ struct KArgs {
   int32_t idx32;      // N < 2^32: node coordinates by 32-bit division
   UDiv div_jk, div_k;  // division by nj * nk and by nk
   int32_t field_layout;  // FieldLayout of the plan's fields
+  MinorQ mq[9];          // kFieldsUniform, FP64: each field's cross-term quotients
 };
 
 // (i, j, k) of flat node index c in a C-order grid (k = 0 in 2D)
@@ -187,13 +197,26 @@ __device__ __forceinline__ double iso_comp(double d, int m0, int m1) {
 // face nodes conduct and both 4-node sides hold at least 3 conducting nodes.
 // Zeros are returned as +0.0; adding or subtracting a signed zero leaves
 // the accumulated weights unchanged, so this matches the reference bits.
-__device__ __forceinline__ void minor_exact(double d, int m0, int m1, int m2,
-                                            int m3, int m4, int m5,
-                                            double q[6]) {
+// T: d's precomputed quotients, or null to divide.
+__device__ __forceinline__ void minor_exact(double d, const MinorQ* T, int m0, int m1, int m2,
+                                            int m3, int m4, int m5, double q[6]) {
   const int mh = m2 + m3 + m4 + m5;
   const int ml = m0 + m1 + m2 + m3;
   if (m2 == 0 || m3 == 0 || mh < 3 || ml < 3) {
     q[0] = q[1] = q[2] = q[3] = q[4] = q[5] = 0.0;
+    return;
+  }
+  if (T) {
+    const int h = mh == 4, l = ml == 4;
+    const double nq = l ? T->nq[1] : T->nq[0];
+    const double pq = h ? T->pq[1] : T->pq[0];
+    const double dc = h ? (l ? T->dc[1][1] : T->dc[1][0]) : (l ? T->dc[0][1] : T->dc[0][0]);
+    q[0] = m0 ? nq : 0.0;
+    q[1] = m1 ? nq : 0.0;
+    q[2] = dc;
+    q[3] = dc;
+    q[4] = m4 ? pq : 0.0;
+    q[5] = m5 ? pq : 0.0;
     return;
   }
   const double dmh = (double)mh, dml = (double)ml;
@@ -235,91 +258,109 @@ __device__ __forceinline__ void wts_aniso2d(double W[9], const Nbr& M,
                                             double dxx0, double dxx1,
                                             double dxy0, double dxy1,
                                             double dyx0, double dyx1,
-                                            double dyy0, double dyy1) {
+                                            double dyy0, double dyy1,
+                                            const MinorQ* Txy = nullptr,
+                                            const MinorQ* Tyx = nullptr) {
 #pragma unroll
   for (int q = 0; q < 9; ++q) W[q] = 0.0;
   double q[6], qm;
   qm = dxx0 * (double)M(-1, 0, 0);
   W[1] += qm; W[4] -= qm;
-  minor_exact(dxy0, M(-1, -1, 0), M(0, -1, 0), M(-1, 0, 0), M(0, 0, 0),
+  minor_exact(dxy0, Txy, M(-1, -1, 0), M(0, -1, 0), M(-1, 0, 0), M(0, 0, 0),
               M(-1, 1, 0), M(0, 1, 0), q);
   W[0] -= q[0]; W[3] -= q[1]; W[1] -= q[2]; W[4] -= q[3]; W[2] -= q[4]; W[5] -= q[5];
   qm = dyy0 * (double)M(0, -1, 0);
   W[3] += qm; W[4] -= qm;
-  minor_exact(dyx0, M(-1, -1, 0), M(-1, 0, 0), M(0, -1, 0), M(0, 0, 0),
+  minor_exact(dyx0, Tyx, M(-1, -1, 0), M(-1, 0, 0), M(0, -1, 0), M(0, 0, 0),
               M(1, -1, 0), M(1, 0, 0), q);
   W[0] -= q[0]; W[1] -= q[1]; W[3] -= q[2]; W[4] -= q[3]; W[6] -= q[4]; W[7] -= q[5];
   qm = dyy1 * (double)M(0, 1, 0);
   W[5] += qm; W[4] -= qm;
-  minor_exact(dyx1, M(-1, 1, 0), M(-1, 0, 0), M(0, 1, 0), M(0, 0, 0),
+  minor_exact(dyx1, Tyx, M(-1, 1, 0), M(-1, 0, 0), M(0, 1, 0), M(0, 0, 0),
               M(1, 1, 0), M(1, 0, 0), q);
   W[2] += q[0]; W[1] += q[1]; W[5] += q[2]; W[4] += q[3]; W[8] += q[4]; W[7] += q[5];
   qm = dxx1 * (double)M(1, 0, 0);
   W[7] += qm; W[4] -= qm;
-  minor_exact(dxy1, M(1, -1, 0), M(0, -1, 0), M(1, 0, 0), M(0, 0, 0),
+  minor_exact(dxy1, Txy, M(1, -1, 0), M(0, -1, 0), M(1, 0, 0), M(0, 0, 0),
               M(1, 1, 0), M(0, 1, 0), q);
   W[6] += q[0]; W[3] += q[1]; W[7] += q[2]; W[4] += q[3]; W[8] += q[4]; W[5] += q[5];
 }
 
 // asymmetric_stencil_3d.py:163-458; d[face][comp]: faces x0 x1 y0 y1 z0 z1,
 // comps: x faces (xx, xy, xz), y faces (yy, yx, yz), z faces (zz, zx, zy)
+// field of d[face][comp] (the field order xx xy xz yx yy yz zx zy zz)
+constexpr int kFaceField[6][3] = {{0, 1, 2}, {0, 1, 2}, {4, 3, 5}, {4, 3, 5}, {8, 6, 7}, {8, 6, 7}};
+
 __device__ __forceinline__ void wts_aniso3d(double W[19], const Nbr& M,
-                                            const double d[6][3]) {
+                                            const double d[6][3],
+                                            const MinorQ* T9 = nullptr) {
 #pragma unroll
   for (int q = 0; q < 19; ++q) W[q] = 0.0;
   double q[6], qm;
   // q (i-1/2, j, k)
   qm = d[0][0] * (double)M(-1, 0, 0);
   W[1] += qm; W[4] -= qm;
-  minor_exact(d[0][1], M(-1, -1, 0), M(0, -1, 0), M(-1, 0, 0), M(0, 0, 0),
+  minor_exact(d[0][1], T9 ? T9 + kFaceField[0][1] : nullptr,
+              M(-1, -1, 0), M(0, -1, 0), M(-1, 0, 0), M(0, 0, 0),
               M(-1, 1, 0), M(0, 1, 0), q);
   W[0] -= q[0]; W[3] -= q[1]; W[1] -= q[2]; W[4] -= q[3]; W[2] -= q[4]; W[5] -= q[5];
-  minor_exact(d[0][2], M(-1, 0, -1), M(0, 0, -1), M(-1, 0, 0), M(0, 0, 0),
+  minor_exact(d[0][2], T9 ? T9 + kFaceField[0][2] : nullptr,
+              M(-1, 0, -1), M(0, 0, -1), M(-1, 0, 0), M(0, 0, 0),
               M(-1, 0, 1), M(0, 0, 1), q);
   W[15] -= q[0]; W[11] -= q[1]; W[1] -= q[2]; W[4] -= q[3]; W[17] -= q[4]; W[12] -= q[5];
   // q (i+1/2, j, k)
   qm = d[1][0] * (double)M(1, 0, 0);
   W[7] += qm; W[4] -= qm;
-  minor_exact(d[1][1], M(1, -1, 0), M(0, -1, 0), M(1, 0, 0), M(0, 0, 0),
+  minor_exact(d[1][1], T9 ? T9 + kFaceField[1][1] : nullptr,
+              M(1, -1, 0), M(0, -1, 0), M(1, 0, 0), M(0, 0, 0),
               M(1, 1, 0), M(0, 1, 0), q);
   W[6] += q[0]; W[3] += q[1]; W[7] += q[2]; W[4] += q[3]; W[8] += q[4]; W[5] += q[5];
-  minor_exact(d[1][2], M(1, 0, -1), M(0, 0, -1), M(1, 0, 0), M(0, 0, 0),
+  minor_exact(d[1][2], T9 ? T9 + kFaceField[1][2] : nullptr,
+              M(1, 0, -1), M(0, 0, -1), M(1, 0, 0), M(0, 0, 0),
               M(1, 0, 1), M(0, 0, 1), q);
   W[16] += q[0]; W[11] += q[1]; W[7] += q[2]; W[4] += q[3]; W[18] += q[4]; W[12] += q[5];
   // q (i, j-1/2, k)
   qm = d[2][0] * (double)M(0, -1, 0);
   W[3] += qm; W[4] -= qm;
-  minor_exact(d[2][1], M(-1, -1, 0), M(-1, 0, 0), M(0, -1, 0), M(0, 0, 0),
+  minor_exact(d[2][1], T9 ? T9 + kFaceField[2][1] : nullptr,
+              M(-1, -1, 0), M(-1, 0, 0), M(0, -1, 0), M(0, 0, 0),
               M(1, -1, 0), M(1, 0, 0), q);
   W[0] -= q[0]; W[1] -= q[1]; W[3] -= q[2]; W[4] -= q[3]; W[6] -= q[4]; W[7] -= q[5];
-  minor_exact(d[2][2], M(0, -1, -1), M(0, 0, -1), M(0, -1, 0), M(0, 0, 0),
+  minor_exact(d[2][2], T9 ? T9 + kFaceField[2][2] : nullptr,
+              M(0, -1, -1), M(0, 0, -1), M(0, -1, 0), M(0, 0, 0),
               M(0, -1, 1), M(0, 0, 1), q);
   W[9] -= q[0]; W[11] -= q[1]; W[3] -= q[2]; W[4] -= q[3]; W[10] -= q[4]; W[12] -= q[5];
   // q (i, j+1/2, k)
   qm = d[3][0] * (double)M(0, 1, 0);
   W[5] += qm; W[4] -= qm;
-  minor_exact(d[3][1], M(-1, 1, 0), M(-1, 0, 0), M(0, 1, 0), M(0, 0, 0),
+  minor_exact(d[3][1], T9 ? T9 + kFaceField[3][1] : nullptr,
+              M(-1, 1, 0), M(-1, 0, 0), M(0, 1, 0), M(0, 0, 0),
               M(1, 1, 0), M(1, 0, 0), q);
   W[2] += q[0]; W[1] += q[1]; W[5] += q[2]; W[4] += q[3]; W[8] += q[4]; W[7] += q[5];
-  minor_exact(d[3][2], M(0, 1, -1), M(0, 0, -1), M(0, 1, 0), M(0, 0, 0),
+  minor_exact(d[3][2], T9 ? T9 + kFaceField[3][2] : nullptr,
+              M(0, 1, -1), M(0, 0, -1), M(0, 1, 0), M(0, 0, 0),
               M(0, 1, 1), M(0, 0, 1), q);
   W[13] += q[0]; W[11] += q[1]; W[5] += q[2]; W[4] += q[3]; W[14] += q[4]; W[12] += q[5];
   // q (i, j, k-1/2)
   qm = d[4][0] * (double)M(0, 0, -1);
   W[11] += qm; W[4] -= qm;
-  minor_exact(d[4][1], M(-1, 0, -1), M(-1, 0, 0), M(0, 0, -1), M(0, 0, 0),
+  minor_exact(d[4][1], T9 ? T9 + kFaceField[4][1] : nullptr,
+              M(-1, 0, -1), M(-1, 0, 0), M(0, 0, -1), M(0, 0, 0),
               M(1, 0, -1), M(1, 0, 0), q);
   W[15] -= q[0]; W[1] -= q[1]; W[11] -= q[2]; W[4] -= q[3]; W[16] -= q[4]; W[7] -= q[5];
-  minor_exact(d[4][2], M(0, -1, -1), M(0, -1, 0), M(0, 0, -1), M(0, 0, 0),
+  minor_exact(d[4][2], T9 ? T9 + kFaceField[4][2] : nullptr,
+              M(0, -1, -1), M(0, -1, 0), M(0, 0, -1), M(0, 0, 0),
               M(0, 1, -1), M(0, 1, 0), q);
   W[9] -= q[0]; W[3] -= q[1]; W[11] -= q[2]; W[4] -= q[3]; W[13] -= q[4]; W[5] -= q[5];
   // q (i, j, k+1/2)
   qm = d[5][0] * (double)M(0, 0, 1);
   W[12] += qm; W[4] -= qm;
-  minor_exact(d[5][1], M(-1, 0, 1), M(-1, 0, 0), M(0, 0, 1), M(0, 0, 0),
+  minor_exact(d[5][1], T9 ? T9 + kFaceField[5][1] : nullptr,
+              M(-1, 0, 1), M(-1, 0, 0), M(0, 0, 1), M(0, 0, 0),
               M(1, 0, 1), M(1, 0, 0), q);
   W[17] += q[0]; W[1] += q[1]; W[12] += q[2]; W[4] += q[3]; W[18] += q[4]; W[7] += q[5];
-  minor_exact(d[5][2], M(0, -1, 1), M(0, -1, 0), M(0, 0, 1), M(0, 0, 0),
+  minor_exact(d[5][2], T9 ? T9 + kFaceField[5][2] : nullptr,
+              M(0, -1, 1), M(0, -1, 0), M(0, 0, 1), M(0, 0, 0),
               M(0, 1, 1), M(0, 1, 0), q);
   W[10] += q[0]; W[3] += q[1]; W[12] += q[2]; W[4] += q[3]; W[14] += q[4]; W[5] += q[5];
 }
@@ -593,7 +634,10 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
         fval_pair<R, 1>(a.f[3], i, j, 0, c, 1, dyy1, dyy0);
       }
       double W[9];
-      wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
+      if constexpr (FL == kFieldsUniform)
+        wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1, &a.mq[1], &a.mq[2]);
+      else
+        wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
       scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
       return u[c - sN - 1] * W[0] + u[c - sN] * W[1] + u[c - sN + 1] * W[2] +
              u[c - 1] * W[3] + uc * W[4] + u[c + 1] * W[5] +
@@ -675,7 +719,7 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
       fval_pair_l<R, 2, FL>(a.f[7], i, j, k, c, 1, d[5][2], d[4][2]);
       if constexpr (EXACT) {
         double W[19];
-        wts_aniso3d(W, M, d);
+        wts_aniso3d(W, M, d, FL == kFieldsUniform ? a.mq : nullptr);
         scale_exact<19>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
         return u[c - si - sj] * W[0] + u[c - si] * W[1] + u[c - si + sj] * W[2] +
                u[c - sj] * W[3] + uc * W[4] + u[c + sj] * W[5] +

@@ -72,6 +72,8 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
     }
   }
   a.field_layout = plan->field_layout;
+  if (FW_PREC == 0 && plan->field_layout == kFieldsUniform)
+    for (int q = 0; q < d.n_fields; ++q) minor_quotients(d.fields[q].value, a.mq[q]);
   if (plan->field_layout == kFieldsK) {
     // every field as its table along the last axis, which the general
     // dispatch (the ECG kernel) reads the same way

@@ -212,11 +212,13 @@ def test_fp32_one_step_with_tabulated_fields_matches_fp64_oracle(gpu, case):
     ("TP062D", "uniform", "fp32"), ("AlievPanfilov2D", "uniform", "fp32"),
     ("AlievPanfilov2D", None, "fp32"), ("TP062D", "uniform", "fp64"),
     ("AlievPanfilov3D", "uniform", "fp32"), ("AlievPanfilov3D", "layered", "fp32"),
-    ("AlievPanfilov3D", "layered", "fp64"), ("AlievPanfilov3D", None, "fp64")])
+    ("AlievPanfilov3D", "layered", "fp64"), ("AlievPanfilov3D", None, "fp64"),
+    ("AlievPanfilov3D", "uniform", "fp64"), ("AlievPanfilov2D", "uniform", "fp64")])
 def test_field_layout_kernels_equal_general_kernel(gpu, name, fibres, precision):
     """The step kernels instantiated for a field layout (every field uniform;
     in 3D, every field uniform or a table along k, as with layered fibres)
-    give the general kernel's bits.  The second tissue's conductivity
+    give the general kernel's bits (FP64: the uniform instance selects the
+    cross-term quotients the host divided out).  The second tissue's conductivity
     differs at the corner node (0, 0(, 0)) only: no updated node's half-step
     fields read it, so the potentials are the same, but its plan falls back
     to the general kernel, which reads its fields by per-field dispatch."""
