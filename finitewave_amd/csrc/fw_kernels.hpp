@@ -155,6 +155,9 @@ struct KArgs {
   UDiv div_jk, div_k;  // division by nj * nk and by nk
   int32_t field_layout;  // FieldLayout of the plan's fields
   MinorQ mq[9];          // kFieldsUniform, FP64: each field's cross-term quotients
+  // kFieldsUniform, FP64, 2D: the final weights (epilogue applied) of a node
+  // whose 3x3 neighbourhood all conducts -- the same for every such node
+  double wfull[9];
 };
 
 // (i, j, k) of flat node index c in a C-order grid (k = 0 in 2D)
@@ -189,7 +192,7 @@ __device__ __forceinline__ int64_t logical_block(uint32_t per_xcd) {
 // ---------------------------------------------------------------------------
 
 // isotropic_stencil_2d.py:135-158: d * m0 * (m0 + (m1 == 0))
-__device__ __forceinline__ double iso_comp(double d, int m0, int m1) {
+__host__ __device__ __forceinline__ double iso_comp(double d, int m0, int m1) {
   return d * (double)m0 * (double)(m0 + (m1 == 0));
 }
 
@@ -198,8 +201,9 @@ __device__ __forceinline__ double iso_comp(double d, int m0, int m1) {
 // Zeros are returned as +0.0; adding or subtracting a signed zero leaves
 // the accumulated weights unchanged, so this matches the reference bits.
 // T: d's precomputed quotients, or null to divide.
-__device__ __forceinline__ void minor_exact(double d, const MinorQ* T, int m0, int m1, int m2,
-                                            int m3, int m4, int m5, double q[6]) {
+__host__ __device__ __forceinline__ void minor_exact(double d, const MinorQ* T, int m0, int m1,
+                                                     int m2, int m3, int m4, int m5,
+                                                     double q[6]) {
   const int mh = m2 + m3 + m4 + m5;
   const int ml = m0 + m1 + m2 + m3;
   if (m2 == 0 || m3 == 0 || mh < 3 || ml < 3) {
@@ -233,7 +237,7 @@ __device__ __forceinline__ void minor_exact(double d, const MinorQ* T, int m0, i
 // is a power of two, inv_dr2 != 0 holds its exact reciprocal and the division
 // is the same multiplication (barring subnormal results).
 template <int K>
-__device__ __forceinline__ void scale_exact(double W[K], int centre, double D,
+__host__ __device__ __forceinline__ void scale_exact(double W[K], int centre, double D,
                                             double dt, double dr2, double inv_dr2) {
   if (inv_dr2 != 0.0) {
 #pragma unroll
@@ -248,13 +252,13 @@ __device__ __forceinline__ void scale_exact(double W[K], int centre, double D,
 // 3x3(x3) neighbourhood of conduct bits, index (di+1)*9 + (dj+1)*3 + (dk+1)
 struct Nbr {
   int m[27];
-  __device__ __forceinline__ int operator()(int di, int dj, int dk) const {
+  __host__ __device__ __forceinline__ int operator()(int di, int dj, int dk) const {
     return m[(di + 1) * 9 + (dj + 1) * 3 + (dk + 1)];
   }
 };
 
 // asymmetric_stencil_2d.py:291-433 (2D: dk = 0 everywhere)
-__device__ __forceinline__ void wts_aniso2d(double W[9], const Nbr& M,
+__host__ __device__ __forceinline__ void wts_aniso2d(double W[9], const Nbr& M,
                                             double dxx0, double dxx1,
                                             double dxy0, double dxy1,
                                             double dyx0, double dyx1,
@@ -603,12 +607,18 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
         fval_pair<R, 1>(a.f[1], i, j, 0, c, 1, dE, dW);
       }
       double W[5];
-      W[0] = iso_comp(dN, mN, mS);
-      W[1] = iso_comp(dW, mW, mE);
-      W[3] = iso_comp(dE, mE, mW);
-      W[4] = iso_comp(dS, mS, mN);
-      W[2] = -(W[0] + W[1] + W[3] + W[4]);
-      scale_exact<5>(W, 2, a.D, a.dtd, a.dr2, a.inv_dr2);
+      if (FL == kFieldsUniform && (mN & mS & mW & mE)) {
+        // every neighbour conducts: the host's weights for such a node
+#pragma unroll
+        for (int q = 0; q < 5; ++q) W[q] = a.wfull[q];
+      } else {
+        W[0] = iso_comp(dN, mN, mS);
+        W[1] = iso_comp(dW, mW, mE);
+        W[3] = iso_comp(dE, mE, mW);
+        W[4] = iso_comp(dS, mS, mN);
+        W[2] = -(W[0] + W[1] + W[3] + W[4]);
+        scale_exact<5>(W, 2, a.D, a.dtd, a.dr2, a.inv_dr2);
+      }
       return u[c - sN] * W[0] + u[c - 1] * W[1] + uc * W[2] + u[c + 1] * W[3] +
              u[c + sN] * W[4];
     } else {  // anisotropic 9-point, exact weights
@@ -634,11 +644,22 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
         fval_pair<R, 1>(a.f[3], i, j, 0, c, 1, dyy1, dyy0);
       }
       double W[9];
-      if constexpr (FL == kFieldsUniform)
-        wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1, &a.mq[1], &a.mq[2]);
-      else
-        wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
-      scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
+      int all = 1;
+#pragma unroll
+      for (int di = -1; di <= 1; ++di)
+#pragma unroll
+        for (int dj = -1; dj <= 1; ++dj) all &= M(di, dj, 0);
+      if (FL == kFieldsUniform && all) {
+        // the whole 3x3 neighbourhood conducts: the host's weights
+#pragma unroll
+        for (int q = 0; q < 9; ++q) W[q] = a.wfull[q];
+      } else {
+        if constexpr (FL == kFieldsUniform)
+          wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1, &a.mq[1], &a.mq[2]);
+        else
+          wts_aniso2d(W, M, dxx0, dxx1, dxy0, dxy1, dyx0, dyx1, dyy0, dyy1);
+        scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
+      }
       return u[c - sN - 1] * W[0] + u[c - sN] * W[1] + u[c - sN + 1] * W[2] +
              u[c - 1] * W[3] + uc * W[4] + u[c + 1] * W[5] +
              u[c + sN - 1] * W[6] + u[c + sN] * W[7] + u[c + sN + 1] * W[8];

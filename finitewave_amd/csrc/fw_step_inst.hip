@@ -94,6 +94,27 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
     const double mant = frexp(d.dr2, &e2);
     a.inv_dr2 = (mant == 0.5 && e2 > -1000 && e2 < 1000) ? ldexp(1.0, 1 - e2) : 0.0;
   }
+  if (FW_PREC == 0 && plan->field_layout == kFieldsUniform && d.ndim == 2) {
+    // the exact weights of a node whose 3x3 neighbourhood all conducts, by
+    // the kernel's own functions (this FP64 unit is built without
+    // contraction); the kernel takes them for every such node
+    Nbr M;
+    for (int q = 0; q < 27; ++q) M.m[q] = (q % 3) == 1;  // the dk = 0 plane
+    const double f0 = d.fields[0].value, f1 = d.fields[1].value;
+    double* W = a.wfull;
+    if (d.stencil == FW_STENCIL_ANISO) {
+      const double f2 = d.fields[2].value, f3 = d.fields[3].value;
+      wts_aniso2d(W, M, f0, f0, f1, f1, f2, f2, f3, f3);
+      scale_exact<9>(W, 4, a.D, a.dtd, a.dr2, a.inv_dr2);
+    } else {
+      W[0] = iso_comp(f0, 1, 1);
+      W[1] = iso_comp(f1, 1, 1);
+      W[3] = iso_comp(f1, 1, 1);
+      W[4] = iso_comp(f0, 1, 1);
+      W[2] = -(W[0] + W[1] + W[3] + W[4]);
+      scale_exact<5>(W, 2, a.D, a.dtd, a.dr2, a.inv_dr2);
+    }
+  }
   a.dt = (Real)d.dt;
   a.s = (UReal)(d.D_model * d.dt / d.dr2);
   for (int q = 0; q < 64; ++q) a.P.v[q] = q < d.n_params ? d.params[q] : 0.0;
