@@ -18,6 +18,13 @@ FP32 path (TP06 calcium and concentrations in FP64) is reported beside it.
 roofline: algorithmic bytes per step / measured time per step against the
 8 TB/s HBM3E peak.  cpu_baseline: the C/OpenMP oracle on the host cores over
 a bounded sample of the same workload.
+
+Beside the headline the line carries the other configurations' legs: the
+512-tissue Mitchell-Schaeffer restitution ensemble (`ensemble`), the
+Poisson-served TP06/Aliev-Panfilov mix (`serving`, with the same requests
+run one after another on the plain FP64 path), the reference's own CPU case
+(`c1`: Aliev-Panfilov 102x102 FP64) and the 3D slab (`c3`: Aliev-Panfilov
+258^3, 19-point stencil, layered fibres, fibrosis, FP32).
 """
 from __future__ import annotations
 
@@ -132,6 +139,119 @@ def gpu_run(precision: str, steps: int, warmup: int):
     eng.close()
     return dict(ms=ms, wall=wall, steps=steps, n_active=n_active, n_nodes=n_nodes,
                 finite=finite, activated=activated)
+
+
+def _engine_steps(model, tracker, steps: int, warmup: int):
+    """Per-step time (us) of `steps` graph-replayed fused steps after
+    `warmup` untimed ones, HIP events on the launch stream; activation
+    tracking on every step."""
+    import torch
+    from finitewave_amd.engine import Engine
+    model.initialize()
+    eng = Engine(model)
+    eng.upload()
+    eng.act[tracker] = torch.from_numpy(tracker._steps_host()).to(eng.dev)
+    eng.run_steps(0, warmup, tracker, np.ones(warmup, np.uint8), use_graph=True)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    eng.run_steps(warmup, steps, tracker, np.ones(steps, np.uint8), use_graph=True)
+    e1.record()
+    torch.cuda.synchronize()
+    finite = bool(torch.isfinite(eng.u_dev()).all().item())
+    eng.close()
+    return e0.elapsed_time(e1) * 1e3 / steps, finite
+
+
+def c1_leg():
+    """The reference's CPU-runnable case: Aliev-Panfilov 2D, 100x100 tissue
+    plus boundary, isotropic 5-point, FP64, planar stimulus on the 5 leftmost
+    tissue columns, 1,000 steps (t_max 10), activation-time tracker.  Its
+    state (0.4 MB) stays in L2: a latency figure, not a roofline one."""
+    import torch
+    import finitewave_amd as fw
+
+    def model():
+        m = fw.AlievPanfilov2D()
+        m.dt, m.dr, m.t_max = 0.01, 0.25, 10
+        m.prog_bar = False
+        m.cardiac_tissue = fw.CardiacTissue2D([102, 102])
+        seq = fw.StimSequence()
+        seq.add_stim(fw.StimVoltageCoord2D(0, 1.0, 0, 102, 0, 6))
+        m.stim_sequence = seq
+        tr = fw.ActivationTime2DTracker()
+        tr.threshold = 0.5
+        ts = fw.TrackerSequence()
+        ts.add_tracker(tr)
+        m.tracker_sequence = ts
+        return m, tr
+
+    m, tr = model()
+    us, finite = _engine_steps(m, tr, 1000, 100)
+    m, tr = model()
+    m.run()
+    torch.cuda.synchronize()
+    walls = []
+    for _ in range(3):
+        w0 = time.perf_counter()
+        m.run()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - w0)
+    return dict(workload="Aliev-Panfilov 2D 102x102 (100x100 tissue), isotropic 5-point, FP64, "
+                         "planar stimulus, activation-time tracker",
+                us_per_step=round(us, 3), gcell=round(10_000 / us / 1e3, 3),
+                run_1000_steps_ms=round(min(walls) * 1e3, 2), finite=finite)
+
+
+def c3_leg(steps: int = 400, warmup: int = 40):
+    """Aliev-Panfilov 3D slab, 256^3 tissue plus boundary, anisotropic
+    19-point stencil, fibres in the xy-plane rotating from -60 deg (first
+    tissue layer in z) to +60 deg (last), D_long 1.0, D_trans 0.111, dr 0.25,
+    dt 0.01, diffuse fibrosis i.i.d. p = 0.2 (numpy seed 0, the Diffuse3D
+    pattern), FP32, planar stimulus on the x < 5 face, activation-time
+    tracker.  Roofline: (u, v read and written in float + the 4-byte
+    activation word) per tissue node + the 1-byte code per node."""
+    import finitewave_amd as fw
+    n = 258
+    tissue = fw.CardiacTissue3D([n, n, n])
+    np.random.seed(0)
+    tissue.add_pattern(fw.Diffuse3DPattern(0, n, 0, n, 0, n, 0.2))
+    k = np.arange(n)
+    phi = np.deg2rad(-60 + 120 * np.clip(k - 1, 0, n - 3) / (n - 3))
+    f = np.zeros((n, n, n, 3))
+    f[..., 0] = np.cos(phi)[None, None, :]
+    f[..., 1] = np.sin(phi)[None, None, :]
+    tissue.fibers = f
+    m = fw.AlievPanfilov3D()
+    m.dt, m.dr, m.t_max = 0.01, 0.25, 1e9
+    m.prog_bar = False
+    m.precision = "fp32"
+    m.cardiac_tissue = tissue
+    st = fw.AsymmetricStencil3D()
+    st.D_al, st.D_ac = 1.0, 0.111
+    m.stencil = st
+    seq = fw.StimSequence()
+    seq.add_stim(fw.StimVoltageCoord3D(0, 1.0, 0, 6, 0, n, 0, n))
+    m.stim_sequence = seq
+    tr = fw.ActivationTime3DTracker()
+    tr.threshold = 0.5
+    ts = fw.TrackerSequence()
+    ts.add_tracker(tr)
+    m.tracker_sequence = ts
+    us, finite = _engine_steps(m, tr, steps, warmup)
+    na = int(np.count_nonzero(tissue.mesh == 1))
+    nodes = int(tissue.mesh.size)
+    bytes_step = (16 + B_TRACK) * na + nodes
+    achieved = bytes_step / (us * 1e-6) / 1e9
+    traffic, source = profiled_traffic("c3")
+    return dict(workload="Aliev-Panfilov 3D 258^3 (256^3 tissue), anisotropic 19-point, fibres "
+                         "-60..60 deg along z, fibrosis p=0.2, FP32, activation-time tracker",
+                nodes=nodes, active=na, steps_timed=steps, us_per_step=round(us, 2),
+                gcell=round(na / us / 1e3, 3), finite=finite,
+                roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                          "traffic": traffic, "traffic_source": source,
+                          "bytes_per_step": bytes_step})
 
 
 def cpu_baseline(steps: int, threads: int):
@@ -354,7 +474,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-fp64-serving", action="store_true")
     ap.add_argument("--per-request", action="store_true",
                     help="list each served request's errors against the FP64 path")
-    ap.add_argument("--legs", default="stencil,ensemble,serving")
+    ap.add_argument("--legs", default="stencil,ensemble,serving,c1,c3")
     args = ap.parse_args(argv)
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -424,6 +544,12 @@ def main(argv=None) -> int:
         line["serving"] = serving_leg(args.serve_requests, not args.no_fp64_serving,
                                       args.per_request)
         print(f"# serving: {line['serving']}", file=sys.stderr, flush=True)
+    if "c1" in legs:
+        line["c1"] = c1_leg()
+        print(f"# c1: {line['c1']}", file=sys.stderr, flush=True)
+    if "c3" in legs:
+        line["c3"] = c3_leg()
+        print(f"# c3: {line['c3']}", file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(args.cpu_steps, threads)
