@@ -144,24 +144,6 @@ __host__ __device__ constexpr int batch_unroll(int variant) {
   return (variant & kBatchVec) ? 4 : 1;
 }
 
-// 4 values of consecutive nodes, element-aligned: one vector load or store
-template <typename T>
-struct Vec4T;
-template <>
-struct Vec4T<float> {
-  typedef float type __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
-};
-template <>
-struct Vec4T<double> {
-  typedef double type __attribute__((ext_vector_type(4))) __attribute__((aligned(8)));
-};
-template <>
-struct Vec4T<int32_t> {
-  typedef int32_t type __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
-};
-template <typename T>
-using Vec4 = typename Vec4T<T>::type;
-typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
 
 template <int MODEL, typename R, int V>
 __global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<MODEL, R>> a) {

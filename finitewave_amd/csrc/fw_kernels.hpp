@@ -566,6 +566,71 @@ __device__ __forceinline__ R flux2d(const KArgs<RA, R>& a, int64_t i, int64_t j,
   }
 }
 
+// The FP32 flux form of the 3D anisotropic 19-point stencil at one node, from
+// its loaded neighbourhood: conduct bits M, neighbour potentials un18 (index
+// (di+1)*9 + (dj+1)*3 + dk+1; corners and centre unused), half-step field
+// values d[face][comp] and the centre potential uc.  The 3D step has no
+// batched counterpart, so the compiler may fuse across statements here (the
+// masked minor flux folds into one fma with the running sum).
+template <typename R>
+__device__ __forceinline__ R flux3d_aniso(const Nbr& M, const R (&un18)[27], const R (&d)[6][3],
+                                          R uc, R s) {
+#pragma clang fp contract(fast)
+  auto U = [&](int di, int dj, int dk) -> R {
+    return M(di, dj, dk) ? un18[(di + 1) * 9 + (dj + 1) * 3 + dk + 1] : R(0);
+  };
+  R fl = R(0);
+  // x0
+  fl += (R)d[0][0] * (R)M(-1, 0, 0) * (U(-1, 0, 0) - uc);
+  fl -= minor_flux<R>((R)d[0][1], M(-1, -1, 0), U(-1, -1, 0), M(0, -1, 0), U(0, -1, 0),
+                      M(-1, 0, 0), U(-1, 0, 0), 1, uc,
+                      M(-1, 1, 0), U(-1, 1, 0), M(0, 1, 0), U(0, 1, 0));
+  fl -= minor_flux<R>((R)d[0][2], M(-1, 0, -1), U(-1, 0, -1), M(0, 0, -1), U(0, 0, -1),
+                      M(-1, 0, 0), U(-1, 0, 0), 1, uc,
+                      M(-1, 0, 1), U(-1, 0, 1), M(0, 0, 1), U(0, 0, 1));
+  // x1
+  fl += (R)d[1][0] * (R)M(1, 0, 0) * (U(1, 0, 0) - uc);
+  fl += minor_flux<R>((R)d[1][1], M(1, -1, 0), U(1, -1, 0), M(0, -1, 0), U(0, -1, 0),
+                      M(1, 0, 0), U(1, 0, 0), 1, uc,
+                      M(1, 1, 0), U(1, 1, 0), M(0, 1, 0), U(0, 1, 0));
+  fl += minor_flux<R>((R)d[1][2], M(1, 0, -1), U(1, 0, -1), M(0, 0, -1), U(0, 0, -1),
+                      M(1, 0, 0), U(1, 0, 0), 1, uc,
+                      M(1, 0, 1), U(1, 0, 1), M(0, 0, 1), U(0, 0, 1));
+  // y0
+  fl += (R)d[2][0] * (R)M(0, -1, 0) * (U(0, -1, 0) - uc);
+  fl -= minor_flux<R>((R)d[2][1], M(-1, -1, 0), U(-1, -1, 0), M(-1, 0, 0), U(-1, 0, 0),
+                      M(0, -1, 0), U(0, -1, 0), 1, uc,
+                      M(1, -1, 0), U(1, -1, 0), M(1, 0, 0), U(1, 0, 0));
+  fl -= minor_flux<R>((R)d[2][2], M(0, -1, -1), U(0, -1, -1), M(0, 0, -1), U(0, 0, -1),
+                      M(0, -1, 0), U(0, -1, 0), 1, uc,
+                      M(0, -1, 1), U(0, -1, 1), M(0, 0, 1), U(0, 0, 1));
+  // y1
+  fl += (R)d[3][0] * (R)M(0, 1, 0) * (U(0, 1, 0) - uc);
+  fl += minor_flux<R>((R)d[3][1], M(-1, 1, 0), U(-1, 1, 0), M(-1, 0, 0), U(-1, 0, 0),
+                      M(0, 1, 0), U(0, 1, 0), 1, uc,
+                      M(1, 1, 0), U(1, 1, 0), M(1, 0, 0), U(1, 0, 0));
+  fl += minor_flux<R>((R)d[3][2], M(0, 1, -1), U(0, 1, -1), M(0, 0, -1), U(0, 0, -1),
+                      M(0, 1, 0), U(0, 1, 0), 1, uc,
+                      M(0, 1, 1), U(0, 1, 1), M(0, 0, 1), U(0, 0, 1));
+  // z0
+  fl += (R)d[4][0] * (R)M(0, 0, -1) * (U(0, 0, -1) - uc);
+  fl -= minor_flux<R>((R)d[4][1], M(-1, 0, -1), U(-1, 0, -1), M(-1, 0, 0), U(-1, 0, 0),
+                      M(0, 0, -1), U(0, 0, -1), 1, uc,
+                      M(1, 0, -1), U(1, 0, -1), M(1, 0, 0), U(1, 0, 0));
+  fl -= minor_flux<R>((R)d[4][2], M(0, -1, -1), U(0, -1, -1), M(0, -1, 0), U(0, -1, 0),
+                      M(0, 0, -1), U(0, 0, -1), 1, uc,
+                      M(0, 1, -1), U(0, 1, -1), M(0, 1, 0), U(0, 1, 0));
+  // z1
+  fl += (R)d[5][0] * (R)M(0, 0, 1) * (U(0, 0, 1) - uc);
+  fl += minor_flux<R>((R)d[5][1], M(-1, 0, 1), U(-1, 0, 1), M(-1, 0, 0), U(-1, 0, 0),
+                      M(0, 0, 1), U(0, 0, 1), 1, uc,
+                      M(1, 0, 1), U(1, 0, 1), M(1, 0, 0), U(1, 0, 0));
+  fl += minor_flux<R>((R)d[5][2], M(0, -1, 1), U(0, -1, 1), M(0, -1, 0), U(0, -1, 0),
+                      M(0, 0, 1), U(0, 0, 1), 1, uc,
+                      M(0, 1, 1), U(0, 1, 1), M(0, 1, 0), U(0, 1, 0));
+  return uc + s * fl;
+}
+
 // R here is the potential's storage type (FP32 path of the millivolt-scale
 // models: double); EXACT selects the reference's FP64 weights, else the flux
 // form evaluated in R; FL: the field layout the caller is specialised for
@@ -763,59 +828,7 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
               const int corner = (di != 0) && (dj != 0) && (dk != 0);
               un18[q] = (corner || q == 13) ? R(0) : u[c + di * si + dj * sj + dk];
             }
-        auto U = [&](int di, int dj, int dk) -> R {
-          return M(di, dj, dk) ? un18[(di + 1) * 9 + (dj + 1) * 3 + dk + 1] : R(0);
-        };
-        R fl = R(0);
-        // x0
-        fl += (R)d[0][0] * (R)M(-1, 0, 0) * (U(-1, 0, 0) - uc);
-        fl -= minor_flux<R>((R)d[0][1], M(-1, -1, 0), U(-1, -1, 0), M(0, -1, 0), U(0, -1, 0),
-                            M(-1, 0, 0), U(-1, 0, 0), 1, uc,
-                            M(-1, 1, 0), U(-1, 1, 0), M(0, 1, 0), U(0, 1, 0));
-        fl -= minor_flux<R>((R)d[0][2], M(-1, 0, -1), U(-1, 0, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(-1, 0, 0), U(-1, 0, 0), 1, uc,
-                            M(-1, 0, 1), U(-1, 0, 1), M(0, 0, 1), U(0, 0, 1));
-        // x1
-        fl += (R)d[1][0] * (R)M(1, 0, 0) * (U(1, 0, 0) - uc);
-        fl += minor_flux<R>((R)d[1][1], M(1, -1, 0), U(1, -1, 0), M(0, -1, 0), U(0, -1, 0),
-                            M(1, 0, 0), U(1, 0, 0), 1, uc,
-                            M(1, 1, 0), U(1, 1, 0), M(0, 1, 0), U(0, 1, 0));
-        fl += minor_flux<R>((R)d[1][2], M(1, 0, -1), U(1, 0, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(1, 0, 0), U(1, 0, 0), 1, uc,
-                            M(1, 0, 1), U(1, 0, 1), M(0, 0, 1), U(0, 0, 1));
-        // y0
-        fl += (R)d[2][0] * (R)M(0, -1, 0) * (U(0, -1, 0) - uc);
-        fl -= minor_flux<R>((R)d[2][1], M(-1, -1, 0), U(-1, -1, 0), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, -1, 0), U(0, -1, 0), 1, uc,
-                            M(1, -1, 0), U(1, -1, 0), M(1, 0, 0), U(1, 0, 0));
-        fl -= minor_flux<R>((R)d[2][2], M(0, -1, -1), U(0, -1, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(0, -1, 0), U(0, -1, 0), 1, uc,
-                            M(0, -1, 1), U(0, -1, 1), M(0, 0, 1), U(0, 0, 1));
-        // y1
-        fl += (R)d[3][0] * (R)M(0, 1, 0) * (U(0, 1, 0) - uc);
-        fl += minor_flux<R>((R)d[3][1], M(-1, 1, 0), U(-1, 1, 0), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, 1, 0), U(0, 1, 0), 1, uc,
-                            M(1, 1, 0), U(1, 1, 0), M(1, 0, 0), U(1, 0, 0));
-        fl += minor_flux<R>((R)d[3][2], M(0, 1, -1), U(0, 1, -1), M(0, 0, -1), U(0, 0, -1),
-                            M(0, 1, 0), U(0, 1, 0), 1, uc,
-                            M(0, 1, 1), U(0, 1, 1), M(0, 0, 1), U(0, 0, 1));
-        // z0
-        fl += (R)d[4][0] * (R)M(0, 0, -1) * (U(0, 0, -1) - uc);
-        fl -= minor_flux<R>((R)d[4][1], M(-1, 0, -1), U(-1, 0, -1), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, 0, -1), U(0, 0, -1), 1, uc,
-                            M(1, 0, -1), U(1, 0, -1), M(1, 0, 0), U(1, 0, 0));
-        fl -= minor_flux<R>((R)d[4][2], M(0, -1, -1), U(0, -1, -1), M(0, -1, 0), U(0, -1, 0),
-                            M(0, 0, -1), U(0, 0, -1), 1, uc,
-                            M(0, 1, -1), U(0, 1, -1), M(0, 1, 0), U(0, 1, 0));
-        // z1
-        fl += (R)d[5][0] * (R)M(0, 0, 1) * (U(0, 0, 1) - uc);
-        fl += minor_flux<R>((R)d[5][1], M(-1, 0, 1), U(-1, 0, 1), M(-1, 0, 0), U(-1, 0, 0),
-                            M(0, 0, 1), U(0, 0, 1), 1, uc,
-                            M(1, 0, 1), U(1, 0, 1), M(1, 0, 0), U(1, 0, 0));
-        fl += minor_flux<R>((R)d[5][2], M(0, -1, 1), U(0, -1, 1), M(0, -1, 0), U(0, -1, 0),
-                            M(0, 0, 1), U(0, 0, 1), 1, uc,
-                            M(0, 1, 1), U(0, 1, 1), M(0, 1, 0), U(0, 1, 0));
-        return uc + a.s * fl;
+        return flux3d_aniso<R>(M, un18, d, uc, a.s);
       }
     }
   }
@@ -832,21 +845,49 @@ constexpr int step_min_waves() {
   return (NDIM == 2 && MODEL == 6) ? (PREC == 0 ? 4 : 5) : 1;
 }
 
-// FL: the plan's field layout (KArgs::field_layout).  The uniform and
-// along-k instances read the field values without the per-field mode
-// dispatch, whose dependent argument loads stalled the waves (1026^2 TP06
-// FP32: 42 -> 40 us per step)
+// 4 values of consecutive nodes, element-aligned: one vector load or store
+template <typename T>
+struct Vec4T;
+template <>
+struct Vec4T<float> {
+  typedef float type __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+};
+template <>
+struct Vec4T<double> {
+  typedef double type __attribute__((ext_vector_type(4))) __attribute__((aligned(8)));
+};
+template <>
+struct Vec4T<int32_t> {
+  typedef int32_t type __attribute__((ext_vector_type(4))) __attribute__((aligned(4)));
+};
+template <typename T>
+using Vec4 = typename Vec4T<T>::type;
+typedef uint32_t __attribute__((aligned(1))) u32_unaligned;
+
+// A point probe's value at node c (pre-step potential uc, post-update state)
+template <typename R, int MODEL, int PREC, typename A>
+__device__ __forceinline__ void probe_node(const A& a, int64_t c, uint8_t code, double uc) {
+  if ((code & 4) && a.probe_row >= 0) {
+    for (int p = 0; p < a.n_probe; ++p) {
+      if (a.probe_idx[p] != c) continue;
+      const int var = a.probe_var[p];
+      const double val = var < 0 ? uc : state_as_double<R, MODEL>(a.st, var, c, PREC);
+      a.probe_buf[(int64_t)a.probe_row * a.n_probe + p] = val;
+    }
+  }
+}
+
+// One node of the fused step: activation time, the update of an interior
+// updated node, probes.  FL: the plan's field layout (KArgs::field_layout).
+// The uniform and along-k instances read the field values without the
+// per-field mode dispatch, whose dependent argument loads stalled the waves
+// (1026^2 TP06 FP32: 42 -> 40 us per step)
 template <int NDIM, int ST, int MODEL, typename R, int PREC, int FL>
-__global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void k_step(
-    const KArgs<R, UType<MODEL, R>> a) {
+__device__ __forceinline__ void step_node(const KArgs<R, UType<MODEL, R>>& a, int64_t c,
+                                          int32_t step) {
   using U = UType<MODEL, R>;
-  const int64_t lb = logical_block(a.per_xcd);
-  if (lb >= a.nblocks) return;
-  const int64_t c = lb * kBlock + threadIdx.x;
-  if (c >= a.N) return;
   const uint8_t code = a.code[c];
   const U uc = a.u[c];
-  const int32_t step = a.step_in ? *a.step_in : a.step;
   if (a.act_on) {
     if (a.act[c] < 0 && uc > a.act_thr) a.act[c] = step;
   }
@@ -872,14 +913,18 @@ __global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void
       }
     }
   }
-  if ((code & 4) && a.probe_row >= 0) {
-    for (int p = 0; p < a.n_probe; ++p) {
-      if (a.probe_idx[p] != c) continue;
-      const int var = a.probe_var[p];
-      const double val = var < 0 ? (double)uc : state_as_double<R, MODEL>(a.st, var, c, PREC);
-      a.probe_buf[(int64_t)a.probe_row * a.n_probe + p] = val;
-    }
-  }
+  probe_node<R, MODEL, PREC>(a, c, code, (double)uc);
+}
+
+template <int NDIM, int ST, int MODEL, typename R, int PREC, int FL>
+__global__ __launch_bounds__(kBlock, (step_min_waves<NDIM, MODEL, PREC>())) void k_step(
+    const KArgs<R, UType<MODEL, R>> a) {
+  const int64_t lb = logical_block(a.per_xcd);
+  if (lb >= a.nblocks) return;
+  const int64_t c = lb * kBlock + threadIdx.x;
+  if (c >= a.N) return;
+  const int32_t step = a.step_in ? *a.step_in : a.step;
+  step_node<NDIM, ST, MODEL, R, PREC, FL>(a, c, step);
   if (a.step_out && c == 0) *a.step_out = step + 1;
 }
 
