@@ -834,15 +834,15 @@ __device__ __forceinline__ R diffuse(const KArgs<RA, R>& a, int64_t i, int64_t j
   }
 }
 
-// Waves per SIMD the compiler must leave room for.  The 2D FP64 TP06 step
-// takes 110 registers under a 128-register budget (4 waves, no spills); left
-// to itself the compiler would choose 3 waves.  The 2D FP32 TP06 step with
-// its neighbourhood loaded early takes 105 registers (4 waves) unless held
-// to 96 (5 waves).  The 3D anisotropic FP64 step would spill 74 registers
-// under a 128-register budget.  Other steps keep the compiler's choice.
+// Waves per SIMD the compiler must leave room for.  The 2D TP06 steps hold 5:
+// FP64 then takes 96 registers and spills 7 (uniform-field instance; 1026^2,
+// 132.3 -> 127.4 us per step against 4 waves without spills; the compiler
+// alone would choose 3); FP32 takes 91-94 without spills.  The 3D
+// anisotropic FP64 step would spill 74 registers under a 128-register
+// budget.  Other steps keep the compiler's choice.
 template <int NDIM, int MODEL, int PREC>
 constexpr int step_min_waves() {
-  return (NDIM == 2 && MODEL == 6) ? (PREC == 0 ? 4 : 5) : 1;
+  return (NDIM == 2 && MODEL == 6) ? 5 : 1;
 }
 
 // 4 values of consecutive nodes, element-aligned: one vector load or store
