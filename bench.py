@@ -478,9 +478,9 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
     legs = set(args.legs.split(","))
     if args.gpus != 1:
-        print("bench.py measures one GPU: every MI355X box here has one device and the "
-              "stepping path has no multi-GPU mode (slab decomposition is a design note); "
-              "run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the stepping path has no multi-GPU mode (the slab "
+              "decomposition across GPUs is designed, not built); run with --gpus 1",
+              file=sys.stderr)
         return 2
     import torch
     if not torch.cuda.is_available():
