@@ -29,6 +29,7 @@ EXPORTS = (
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
     "fw_stim_apply", "fw_act_time_update", "fw_lat_update", "fw_spiral_tips", "fw_ecg",
+    "fw_fibrosis",
     "fw_batch_create", "fw_batch_destroy",
     "fw_batch_set_tissue", "fw_batch_set_active", "fw_batch_release", "fw_batch_block_nodes",
     "fw_batch_variants", "fw_batch_set_variant", "fw_batch_get_variant",
@@ -104,6 +105,8 @@ def lib():
     L.fw_lat_update.argtypes = [vp, vp, vp, vp, i32, dbl, i32, vp, vp]
     L.fw_spiral_tips.argtypes = [vp, vp, vp, dbl, vp, i32, vp, vp]
     L.fw_ecg.argtypes = [vp, vp, vp, i32, dbl, vp, vp]
+    L.fw_fibrosis.argtypes = [vp, i32, POINTER(i64), POINTER(i64), POINTER(i64), vp, dbl,
+                              ctypes.c_uint64, vp]
     for name in EXPORTS:
         if name not in ("fw_last_error",):
             getattr(L, name).restype = ctypes.c_int

@@ -131,6 +131,42 @@ __global__ void k_check_codes(const uint8_t* __restrict__ code, int64_t ni, int6
   if (found) atomicOr(bad, found);
 }
 
+// the seed mixed once on the host (SplitMix64's output function, below)
+static uint64_t host_splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// SplitMix64's output function (Steele, Lea and Flood, "Fast splittable
+// pseudorandom number generators", OOPSLA 2014): a bijective 64-bit mix, here
+// applied to a counter (node or block index) and a mixed seed.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9e3779b97f4a7c15ull;
+  x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+  x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+  return x ^ (x >> 31);
+}
+
+// Mesh codes of the region [lo, hi) of a C-order grid (shape n[3], 2D:
+// n[2] = 1): 2 where the node's (or its block's) draw u in [0, 1) is
+// <= density, else 1; grid-stride over the region's nodes.
+__global__ void k_fibrosis(int8_t* __restrict__ mesh, int64_t n1, int64_t n2, int64_t lo0,
+                           int64_t lo1, int64_t lo2, int64_t r1, int64_t r2, int64_t count,
+                           int64_t b0, int64_t b1, int64_t b2, int64_t nb1, int64_t nb2,
+                           int32_t diffuse, double density, uint64_t seed_mix) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < count;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t ri = r / (r1 * r2), rj = (r / r2) % r1, rk = r % r2;
+    const int64_t c = ((lo0 + ri) * n1 + (lo1 + rj)) * n2 + (lo2 + rk);
+    const uint64_t key =
+        diffuse ? (uint64_t)c : (uint64_t)(((ri / b0) * nb1 + rj / b1) * nb2 + rk / b2);
+    const double u = (double)(splitmix64(key ^ seed_mix) >> 11) * 0x1.0p-53;
+    mesh[c] = u <= density ? 2 : 1;
+  }
+}
+
 }  // namespace fw
 
 using namespace fw;
@@ -619,6 +655,42 @@ int fw_lat_update(fw_plan* plan, const void* u, int32_t* state, int32_t* steps,
   }
   return launch_lat(u_is_f64(plan->desc.model, plan->desc.precision), u, state, steps, capacity,
                     thr, step, plan->N, overflow, as_stream(stream));
+}
+
+int fw_fibrosis(int8_t* mesh, int32_t ndim, const int64_t* shape, const int64_t* lo,
+                const int64_t* hi, const int64_t* block, double density, uint64_t seed,
+                void* stream) {
+  if (!mesh || !shape || !lo || !hi || (ndim != 2 && ndim != 3)) {
+    set_error("fw_fibrosis: bad arguments");
+    return FW_E_INVALID;
+  }
+  int64_t n[3] = {1, 1, 1}, l[3] = {0, 0, 0}, h[3] = {1, 1, 1}, b[3] = {1, 1, 1};
+  for (int a = 0; a < ndim; ++a) {
+    n[a] = shape[a];
+    l[a] = lo[a];
+    h[a] = hi[a];
+    if (block) b[a] = block[a];
+    if (n[a] < 1 || l[a] < 0 || l[a] > h[a] || h[a] > n[a] || b[a] < 1) {
+      set_error("fw_fibrosis: axis %d: shape %lld, region [%lld, %lld), block %lld", a,
+                (long long)n[a], (long long)l[a], (long long)h[a], (long long)b[a]);
+      return FW_E_INVALID;
+    }
+  }
+  const int64_t r[3] = {h[0] - l[0], h[1] - l[1], h[2] - l[2]};
+  const int64_t count = r[0] * r[1] * r[2];
+  if (count == 0) return FW_OK;
+  const int64_t nb1 = (r[1] + b[1] - 1) / b[1], nb2 = (r[2] + b[2] - 1) / b[2];
+  const int64_t blocks = (count + 255) / 256;
+  hipLaunchKernelGGL(k_fibrosis, dim3((unsigned)(blocks < 65536 ? blocks : 65536)), dim3(256),
+                     0, as_stream(stream), mesh, n[1], n[2], l[0], l[1], l[2], r[1], r[2],
+                     count, b[0], b[1], b[2], nb1, nb2, block ? 0 : 1, density,
+                     host_splitmix64(seed));
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    set_error("fw_fibrosis: launch failed: %s", hipGetErrorString(e));
+    return FW_E_HIP;
+  }
+  return FW_OK;
 }
 
 int fw_spiral_tips(fw_plan* plan, const void* u_prev, const void* u, double thr, double* tips,

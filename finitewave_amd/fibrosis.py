@@ -7,6 +7,11 @@ one ``random.random()`` per block for the structural one.  A user's seeded
 meshes are therefore the reference's, bit for bit; a device generator could
 not reproduce those streams, and the generation is one-off.
 
+With ``device=True`` the pattern is drawn on the GPU instead
+(``fw_fibrosis``): the same distribution, from a counter-based generator keyed
+by ``seed`` and the node's (or block's) index, for large meshes where the host
+loop is slow; it does not reproduce numpy's or Python's streams.
+
 Reference: core/fibrosis/fibrosis_pattern.py, cpuwave2D/fibrosis/*.py,
 cpuwave3D/fibrosis/*.py.
 """
@@ -23,22 +28,58 @@ __all__ = ["FibrosisPattern", "Diffuse2DPattern", "Structural2DPattern",
 class FibrosisPattern:
     """A pattern that produces mesh codes (core/fibrosis/fibrosis_pattern.py:4-51)."""
 
-    def generate(self, shape=None, mesh=None):
+    def generate(self, shape=None, mesh=None, *, device=False, seed=0):
         """Returns mesh codes: a new all-tissue mesh of `shape` with the
         pattern in its region, or `mesh` itself with the pattern written
-        into its region.  `shape` wins when both are given."""
+        into its region.  `shape` wins when both are given.  device: draw
+        on the GPU, keyed by `seed`."""
         if shape is None and mesh is None:
             raise ValueError("Either shape or mesh must be provided.")
         if shape is not None:
             mesh = np.ones(shape, dtype=np.int8)
+        if device:
+            self._device_generate(mesh, seed)
+            return mesh
         codes = self._codes(mesh.shape)
         region = self._region()
         mesh[region] = codes[region]
         return mesh
 
-    def apply(self, cardiac_tissue):
+    def apply(self, cardiac_tissue, *, device=False, seed=0):
         """Writes the pattern into the tissue's mesh (through its setter)."""
-        cardiac_tissue.mesh = self.generate(mesh=cardiac_tissue.mesh)
+        cardiac_tissue.mesh = self.generate(mesh=cardiac_tissue.mesh, device=device, seed=seed)
+
+    def _blocks(self):
+        return None  # one draw per node
+
+    def _device_generate(self, mesh, seed):
+        import ctypes
+
+        from . import _native
+        torch = _native.require_gpu()
+        if mesh.dtype != np.int8 or mesh.ndim not in (2, 3):
+            raise ValueError("device patterns take a 2D or 3D int8 mesh")
+        lo, hi = [], []
+        for sl, n in zip(self._region(), mesh.shape):
+            a, b, step = sl.indices(n)
+            if step != 1:
+                raise ValueError("pattern regions have unit step")
+            lo.append(a)
+            hi.append(max(a, b))
+        dev = torch.device("cuda", torch.cuda.current_device())
+        t = torch.from_numpy(np.ascontiguousarray(mesh)).to(dev)
+        def arr(v):
+            return (ctypes.c_int64 * len(v))(*v)
+
+        shape_a, lo_a, hi_a = arr(mesh.shape), arr(lo), arr(hi)
+        blocks = self._blocks()
+        block_a = arr(blocks) if blocks is not None else None
+        _native.check(_native.lib().fw_fibrosis(
+            ctypes.c_void_p(t.data_ptr()), mesh.ndim, shape_a, lo_a, hi_a,
+            ctypes.cast(block_a, ctypes.c_void_p) if block_a is not None else None,
+            float(self.density), ctypes.c_uint64(int(seed) & (2**64 - 1)),
+            ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)), "fw_fibrosis")
+        mesh[...] = t.cpu().numpy()
 
     def _region(self):
         raise NotImplementedError
@@ -55,6 +96,9 @@ class _Diffuse(FibrosisPattern):
 
 
 class _Structural(FibrosisPattern):
+    def _blocks(self):
+        return tuple(int(x) for x in self._lengths())
+
     def _codes(self, shape):
         # blocks on a lattice anchored at the region's low corner, one draw
         # per block in C order, clipped at the region's high edges

@@ -252,6 +252,24 @@ int fw_spiral_tips(fw_plan* plan, const void* u_prev, const void* u, double thr,
 int fw_ecg(fw_plan* plan, const void* u, const double* coords, int32_t n_points,
            double dr, double* out, void* stream);
 
+/* ---- tissue set-up ---------------------------------------------------------
+ * Fibrosis generated on the device, the FibrosisPattern classes' work
+ * (core/fibrosis/fibrosis_pattern.py:17-51): writes mesh code 1 (tissue) or 2
+ * (fibrosis) into the region [lo, hi) of an int8 mesh with `ndim` axes
+ * `shape` (C order, device memory) and leaves the nodes outside it as they
+ * are.  block null: each node is fibrosis with probability `density`
+ * (Diffuse2DPattern / Diffuse3DPattern, cpuwave2D/fibrosis/
+ * diffuse_2d_pattern.py:86-87); otherwise blocks of block[a] nodes along each
+ * axis, tiling the region from lo and clipped at hi, are fibrosis with
+ * probability `density` (Structural2DPattern / Structural3DPattern,
+ * cpuwave2D/fibrosis/structural_2d_pattern.py:97-116).  The draws come from a
+ * counter-based generator keyed by `seed` and the node's flat index (or the
+ * block's index), so the mesh does not depend on the launch; they are not
+ * numpy's or Python's streams, which only the host patterns reproduce. */
+int fw_fibrosis(int8_t* mesh, int32_t ndim, const int64_t* shape, const int64_t* lo,
+                const int64_t* hi, const int64_t* block, double density, uint64_t seed,
+                void* stream);
+
 /* ---- batches: many 2D tissues of one model, one launch per step ----------
  * No counterpart in the reference, which runs one tissue per process; this
  * is the ensemble / serving path built on the same fused step.  Each tissue

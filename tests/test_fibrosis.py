@@ -5,8 +5,9 @@ tests/test_fibrosis_3d.py: the same shapes, regions, densities and
 tolerances).  The rest fix what those leave open: which generator is drawn,
 how many draws and in which order (so a seeded mesh is the reference's), the
 region and the clipping of the last blocks, and ``apply`` through the
-tissue's mesh setter.  The last test steps a pattern-built tissue on the GPU
-against the oracle."""
+tissue's mesh setter.  Two GPU tests step pattern-built tissues against the
+oracle; the last two hold the patterns drawn on the device (fw_fibrosis) to
+the reference's own statistical windows, their seed, region and blocks."""
 from __future__ import annotations
 
 import random
@@ -254,3 +255,68 @@ def test_fibrosis_added_by_a_command_mid_run_matches_oracle(gpu):
     model.run(initialize=False)
     sim.run(14, initialize=False)
     assert np.array_equal(model.u, sim.u)
+
+
+# -- generation on the device --------------------------------------------------
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["diffuse2d", "structural2d", "diffuse3d", "structural3d"])
+def test_device_patterns_meet_the_reference_windows(gpu, case):
+    """The reference's four fibrosis tests (same shapes, regions, densities,
+    tolerances) with the pattern drawn on the device."""
+    if case == "diffuse2d":
+        pat, shape, region, dens, tol = (fw.Diffuse2DPattern(0.3, 100, 900, 200, 800),
+                                         (1000, 1000), np.s_[100:900, 200:800], 0.3, 0.01)
+    elif case == "structural2d":
+        pat, shape, region, dens, tol = (fw.Structural2DPattern(0.4, 5, 4, 100, 900, 200, 800),
+                                         (1000, 1000), np.s_[100:900, 200:800], 0.4, 0.05)
+    elif case == "diffuse3d":
+        pat, shape, region, dens, tol = (fw.Diffuse3DPattern(10, 90, 20, 80, 30, 70, 0.3),
+                                         (100, 100, 100), np.s_[10:90, 20:80, 30:70], 0.3, 0.01)
+    else:
+        pat, shape, region, dens, tol = (fw.Structural3DPattern(10, 90, 20, 80, 30, 70, 0.4,
+                                                                5, 4, 3),
+                                         (100, 100, 100), np.s_[10:90, 20:80, 30:70], 0.4, 0.05)
+    res = pat.generate(shape=shape, device=True, seed=0)
+    assert res.shape == shape and res.dtype == np.int8 and np.all(np.isin(res, [1, 2]))
+    assert abs(_ratio(res, region) - dens) < tol
+    outside = np.ones(shape, dtype=bool)
+    outside[region] = False
+    assert np.all(res[outside] == 1)
+
+
+@pytest.mark.gpu
+def test_device_patterns_are_keyed_by_seed_and_keep_blocks_whole(gpu):
+    """The same seed gives the same mesh, another seed another; the nodes
+    outside the region keep their codes; every structural block is all
+    tissue or all fibrosis, clipped at the region's high edges."""
+    pat = fw.Diffuse2DPattern(0.5, 3, 40, 5, 33)
+    a = pat.generate(shape=(44, 37), device=True, seed=7)
+    b = pat.generate(shape=(44, 37), device=True, seed=7)
+    c = pat.generate(shape=(44, 37), device=True, seed=8)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    mesh = np.zeros((44, 37), dtype=np.int8)
+    out = pat.generate(mesh=mesh, device=True, seed=7)
+    assert out is mesh
+    assert np.array_equal(mesh[3:40, 5:33], a[3:40, 5:33])
+    rest = np.ones(mesh.shape, dtype=bool)
+    rest[3:40, 5:33] = False
+    assert np.all(mesh[rest] == 0)
+    # structural 3D: blocks 4 x 3 x 5 from (2, 1, 0), region ends cut the last ones
+    lo, hi, ln = (2, 1, 0), (19, 15, 13), (4, 3, 5)
+    s = fw.Structural3DPattern(lo[0], hi[0], lo[1], hi[1], lo[2], hi[2], 0.5, *ln)
+    m = s.generate(shape=(20, 16, 14), device=True, seed=3)
+    n_blocks, n_fib = 0, 0
+    for bi in range(lo[0], hi[0], ln[0]):
+        for bj in range(lo[1], hi[1], ln[1]):
+            for bk in range(lo[2], hi[2], ln[2]):
+                blk = m[bi:min(bi + ln[0], hi[0]), bj:min(bj + ln[1], hi[1]),
+                        bk:min(bk + ln[2], hi[2])]
+                assert np.all(blk == blk.flat[0])
+                n_blocks += 1
+                n_fib += int(blk.flat[0] == 2)
+    assert 0 < n_fib < n_blocks
+    # a tissue takes the device pattern through apply(), keeping its zero boundary
+    tissue = fw.CardiacTissue2D([30, 30])
+    fw.Diffuse2DPattern(0.3).apply(tissue, device=True, seed=1)
+    assert np.all(tissue.mesh[0] == 0) and np.any(tissue.mesh[1:-1, 1:-1] == 2)
