@@ -4,8 +4,10 @@
  * Plain C: pointers, sizes, status codes.  Every array argument is a device
  * pointer (HIP), C-order over the grid shape, unless stated otherwise.  No
  * entry point allocates device memory except fw_plan_create (a small
- * parameter block and, for an FP32 plan whose potential is float, float
- * copies of its tabulated half-step fields, read once at creation),
+ * parameter block; for an FP32 plan whose potential is float, float copies
+ * of its tabulated half-step fields; for a 3D plan whose fields are uniform
+ * or tables along the last axis, each field as such a table in double and
+ * float -- all read once at creation),
  * fw_batch_create (the tissue descriptor table) and fw_batch_set_active (the
  * block table, when it grows).  `stream` is a
  * hipStream_t passed as void* (NULL = the default stream).  All launches are
@@ -44,6 +46,11 @@
  *   fw_spiral_tips      <- _track_tip_line (spiral_wave_core_2d_tracker.py:211-249)
  *   fw_ecg              <- ECG2DTracker.calc_ecg + compute_ecg
  *                          (ecg_2d_tracker.py:61-151)
+ *   fw_fibrosis         <- Diffuse2D/3DPattern._generate and
+ *                          Structural2D/3DPattern._generate
+ *                          (diffuse_2d_pattern.py:86-87,
+ *                          structural_2d_pattern.py:97-116): the same
+ *                          distribution, drawn on the device
  *   fw_batch_*          <- no counterpart (the reference steps one tissue per
  *                          process); many CardiacModel2D.run() calls of one
  *                          model sharing each step's launch.
