@@ -320,3 +320,37 @@ def test_device_patterns_are_keyed_by_seed_and_keep_blocks_whole(gpu):
     tissue = fw.CardiacTissue2D([30, 30])
     fw.Diffuse2DPattern(0.3).apply(tissue, device=True, seed=1)
     assert np.all(tissue.mesh[0] == 0) and np.any(tissue.mesh[1:-1, 1:-1] == 2)
+
+
+def test_device_entry_point_checks_arguments_before_any_launch():
+    """fw_fibrosis refuses a bad rank, a region outside the grid or inverted,
+    and a block length below 1, with a negative status and a message naming
+    the axis; an empty region returns 0.  These paths return before any
+    device call, so they run without a GPU (the mesh pointer is never read)."""
+    import ctypes
+
+    from finitewave_amd import _native
+    L = _native.lib()
+
+    def arr(v):
+        return (ctypes.c_int64 * len(v))(*v)
+
+    mesh = ctypes.c_void_p(256)
+
+    def call(ndim, shape, lo, hi, block=None):
+        blk = ctypes.cast(arr(block), ctypes.c_void_p) if block is not None else None
+        return L.fw_fibrosis(mesh, ndim, arr(shape), arr(lo), arr(hi), blk, 0.3,
+                             ctypes.c_uint64(5), None)
+
+    assert call(4, [5, 5, 5], [0, 0, 0], [5, 5, 5]) < 0
+    assert b"bad arguments" in L.fw_last_error()
+    assert call(2, [5, 5], [0, 1], [5, 6]) < 0          # past the grid's end
+    assert b"axis 1" in L.fw_last_error()
+    assert call(2, [5, 5], [3, 0], [2, 5]) < 0          # lo > hi
+    assert b"axis 0" in L.fw_last_error()
+    assert call(2, [5, 5], [-1, 0], [2, 5]) < 0         # negative start
+    assert b"axis 0" in L.fw_last_error()
+    assert call(3, [5, 5, 5], [0, 0, 0], [5, 5, 5], [1, 0, 1]) < 0
+    assert b"axis 1" in L.fw_last_error() and b"block 0" in L.fw_last_error()
+    assert call(2, [5, 5], [2, 0], [2, 5]) == 0          # empty region: nothing to draw
+    assert call(3, [5, 5, 5], [0, 0, 4], [5, 5, 4], [2, 2, 2]) == 0
