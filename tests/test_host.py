@@ -360,3 +360,15 @@ def test_batch_launch_variants_per_model():
     arr = (ctypes.c_int32 * 2)()
     assert L.fw_batch_variants(2, arr, 2) == 8 and list(arr) == [0, 1]  # count beyond capacity
     assert L.fw_batch_variants(99, None, 0) == -1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    why, before any device work."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in (r.stdout + r.stderr)
+    assert r.stdout.strip() == "" or not r.stdout.lstrip().startswith("{")
