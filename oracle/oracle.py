@@ -181,7 +181,10 @@ MODELS = {
 
 @dataclass
 class OStim:
-    """kind 'voltage' or 'current'; region: tuple of slices or bool matrix."""
+    """kind 'voltage' or 'current'; region: tuple of slices or bool matrix.
+    values: a voltage per application (StimVoltageListMatrix3D.stimulate,
+    cpuwave3D/stimulation/stim_voltage_list_matrix_3d.py:60-72: the k-th
+    application sets values[k])."""
     kind: str
     t: float
     value: float
@@ -189,11 +192,17 @@ class OStim:
     duration: float = 0.0
     u_max: float | None = None
     passed: bool = False
+    values: list | None = None
+    count: int = 0
 
     def apply(self, sim: "OracleSim") -> None:
         if isinstance(self.region, np.ndarray):
             mask = (self.region > 0) & (sim.mesh == 1)
             if self.kind == "voltage":
+                if self.values is not None:
+                    sim.u[mask] = self.values[self.count]
+                    self.count += 1
+                    return
                 sim.u[mask] = self.value
             else:
                 sim.u[mask] += sim.dt * self.value

@@ -11,6 +11,9 @@ terms).
 """
 from __future__ import annotations
 
+import json
+from pathlib import Path
+
 import numpy as np
 import pytest
 
@@ -227,22 +230,63 @@ def test_current_stimulus_fencepost():
     assert count["n"] == 51
 
 
-def test_model_tables_match_the_product():
-    # the product's parameter/initial-value tables equal the oracle's
+GOLDEN = Path(__file__).resolve().parent / "golden"
+ORACLE_NAMES = {"AlievPanfilov": "aliev_panfilov", "Barkley": "barkley",
+                "MitchellSchaeffer": "mitchell_schaeffer", "FentonKarma": "fenton_karma",
+                "BuenoOrovio": "bueno_orovio", "LuoRudy91": "luo_rudy91", "TP06": "tp06",
+                "Courtemanche": "courtemanche"}
+
+
+def _reference_tables():
+    # written by tests/golden/extract_reference_tables.py from the reference's
+    # source text (ast, nothing imported)
+    return json.loads((GOLDEN / "reference_tables.json").read_text())
+
+
+def test_reference_tables_cover_every_model_class():
+    tables = _reference_tables()
+    assert sorted(tables) == sorted(f"{k}{d}" for k in ORACLE_NAMES for d in ("2D", "3D"))
+    for cls, t in tables.items():
+        # the kernel's own parameter list: u_new, u, the state slots,
+        # indexes, dt, the scalars -- the call fills every slot
+        assert t["kernel_params"][:2] == ["u_new", "u"], cls
+        assert len(t["kernel_params"]) == 4 + len(t["state_args"]) + len(t["scalars"]), cls
+        assert t["state_vars"] == ["u"] + sorted(t["state_args"], key=t["state_vars"].index), cls
+    # Courtemanche passes self.xr into the kernel's xs slot and vice versa
+    # (courtemanche_2d.py:149 vs :569; courtemanche_3d.py:57 vs :90)
+    for d in ("2D", "3D"):
+        t = tables[f"Courtemanche{d}"]
+        k = t["state_args"].index("xr")
+        assert t["state_slots"][k] == "xs" and t["state_args"][k + 1] == "xs"
+        assert t["state_slots"][k + 1] == "xr"
+
+
+@pytest.mark.parametrize("cls_name", sorted(f"{k}{d}" for k in ORACLE_NAMES for d in ("2D", "3D")))
+def test_oracle_and_product_tables_equal_the_reference(cls_name):
+    """Parameters (in the kernel's argument order), state arrays (in the
+    call's order), initial values, D_model and the u_new initialisation of
+    the oracle and of the product equal what the reference's source holds."""
     import finitewave_amd as fw
-    classes = {"aliev_panfilov": fw.AlievPanfilov2D, "barkley": fw.Barkley2D,
-               "mitchell_schaeffer": fw.MitchellSchaeffer2D, "fenton_karma": fw.FentonKarma2D,
-               "bueno_orovio": fw.BuenoOrovio2D, "luo_rudy91": fw.LuoRudy912D,
-               "tp06": fw.TP062D, "courtemanche": fw.Courtemanche2D}
-    for name, cls in classes.items():
-        m = cls()
-        spec = MODELS[name]
-        assert m._param_values() == [float(v) for _, v in spec["params"]], name
-        assert m._state_call_order() == [n for n, _ in spec["state"]], name
-        assert m.D_model == spec["D"] and m.init_u == spec["init_u"], name
-        inits = dict(spec["state"])
-        for attr, init_attr, _ in cls._state:
-            assert getattr(m, init_attr) == inits[attr], (name, attr)
+    t = _reference_tables()[cls_name]
+    spec = MODELS[ORACLE_NAMES[cls_name[:-2]]]
+    # oracle
+    assert [float(v) for _, v in spec["params"]] == [float(v) for v in t["scalars"]]
+    assert [n for n, _ in spec["state"]] == t["state_args"]
+    assert [float(v) for _, v in spec["state"]] == [float(v) for v in t["init"]]
+    assert spec["D"] == t["D_model"] and spec["init_u"] == t["init_u"]
+    assert spec["copy_u_new"] == t["copy_u_new"]
+    # product
+    m = getattr(fw, cls_name)()
+    assert m._param_values() == [float(v) for v in t["scalars"]]
+    assert [getattr(m, a) for a in t["scalar_args"]] == t["scalars"]
+    assert m._state_call_order() == t["state_args"]
+    assert m.D_model == t["D_model"] and m.init_u == t["init_u"]
+    assert m.state_vars == t["state_vars"]
+    assert m.npfloat == t["npfloat"]
+    assert type(m)._copy_u_new == t["copy_u_new"]
+    for (attr, init_attr, _), want in zip(
+            sorted(type(m)._state, key=lambda s: t["state_args"].index(s[0])), t["init"]):
+        assert getattr(m, init_attr) == want, (cls_name, attr)
 
 
 def _minor_flux(d, m0, u0, m1, u1, m2, u2, m3, u3, m4, u4, m5, u5):
