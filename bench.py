@@ -47,12 +47,15 @@ N_SIDE = 1024
 FIBER_DEG = 30.0
 DR = 0.25  # 0.025 cm in the model's millimetres
 
-# bytes each step must move per active node: state read + state written
-# (TP06 reads u and 18 state arrays and writes u_new and 18 arrays; FP32 path:
-# 12 gates as FP32; u/u_new, the sodium gates h, j and cai/casr/cass/nai/Ki as
-# FP64), plus the 1-byte node code per node and the 4-byte activation-time word
-# per node
-B_STATE = {"fp64": 304, "fp32": 212}
+# algorithmic bytes per active node and step: the state the reference's step
+# reads + the state it writes (TP06 reads u and 18 state arrays, writes u_new,
+# 17 of them and oo; cai is read-only).  FP64: 38 arrays x 8 B.  Mixed FP32
+# form: FP32 state with cai, casr, cass, nai, Ki in FP64 = 188 B.  Plus the
+# 1-byte node code per node and the 4-byte activation word per active node.
+B_STATE = {"fp64": 304, "fp32": 188}
+# bytes the FP32 kernel stores per node: it also keeps u, h and j in FP64
+# (a float potential and float slow sodium gates stop moving near rest)
+B_STORED = {"fp64": 304, "fp32": 212}
 B_TRACK = 4
 
 
@@ -95,6 +98,12 @@ def build_model(precision: str):
 
 
 def gpu_run(precision: str, steps: int, warmup: int):
+    """`warmup` untimed steps from t = 0, then `steps` timed ones, as
+    model.run() performs them: the S1 current (steps 0-100) on the device
+    schedule, applied by the step kernels, activation tracking on every
+    step.  The graphs the timed call replays are captured before the start
+    event; the timed region holds the replays and the stimulus launch of its
+    first step only."""
     import torch
     from finitewave_amd.engine import Engine
     model, tracker = build_model(precision)
@@ -102,30 +111,18 @@ def gpu_run(precision: str, steps: int, warmup: int):
     eng = Engine(model)
     eng.upload()
     eng.act[tracker] = torch.from_numpy(tracker._steps_host()).to(eng.dev)
-    stim = model.stim_sequence.sequence[0]
-    # warm-up: the S1 stimulus window then plain steps, all untimed
-    t = 0.0
-    s = 0
-    while s < warmup:
-        if t >= stim.t and not stim.passed:
-            stim._apply_device(model, eng)
-            model.t = t
-            stim.update_status(model)
-            eng.run_steps(s, 1, tracker, np.ones(1, np.uint8), use_graph=False)
-            s += 1
-            t += model.dt
-            continue
-        n = warmup - s
-        eng.run_steps(s, n, tracker, np.ones(n, np.uint8), use_graph=True)
-        for _ in range(n):
-            t += model.dt
-        s += n
+    stims = list(model.stim_sequence.sequence)
+    model._schedule_stims(eng, stims, warmup + steps)
+    s1 = [(e.step_lo, e.step_hi) for e in eng.stim_sched]
+    if warmup:
+        eng.run_steps(0, warmup, tracker, np.ones(warmup, np.uint8), use_graph=True)
+    eng.prepare_graphs(steps - (1 if eng.swapped else 0), tracker)
     torch.cuda.synchronize()
     flags = np.ones(steps, np.uint8)
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
     start.record()
-    eng.run_steps(s, steps, tracker, flags, use_graph=True)
+    eng.run_steps(warmup, steps, tracker, flags, use_graph=True)
     end.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0
@@ -136,9 +133,10 @@ def gpu_run(precision: str, steps: int, warmup: int):
     activated = int((act >= 0).sum().item())
     n_active = int(len(model.cardiac_tissue.myo_indexes))
     n_nodes = int(model.cardiac_tissue.mesh.size)
+    stimulated = sum(max(0, min(hi, warmup + steps - 1) - max(lo, warmup) + 1) for lo, hi in s1)
     eng.close()
     return dict(ms=ms, wall=wall, steps=steps, n_active=n_active, n_nodes=n_nodes,
-                finite=finite, activated=activated)
+                finite=finite, activated=activated, s1_steps=s1, stimulated_timed=stimulated)
 
 
 def _engine_steps(model, tracker, steps: int, warmup: int):
@@ -254,11 +252,25 @@ def c3_leg(steps: int = 400, warmup: int = 40):
                           "bytes_per_step": bytes_step})
 
 
+def _cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(steps: int, threads: int):
     """The oracle (C restatement of the reference's numba kernels, same data
-    layout: (*shape, 9) FP64 weights, int64 active-index list, one diffusion
-    and one ionic sweep per step) on `threads` host threads."""
+    layout: (*shape, K) FP64 weights, int64 active-index list, one diffusion
+    and one ionic sweep per step) on `threads` host threads: the headline
+    configuration (a bounded sample of TP06 1026^2 steps), the reference's own
+    CPU case in full (Aliev-Panfilov 102^2, 1,000 steps), and samples of the
+    3D slab and of the restitution ensemble from which their rates follow."""
     from oracle.oracle import OracleSim, OStim, set_threads
+    used = set_threads(threads)
     n = N_SIDE + 2
     mesh = np.ones((n, n), dtype=np.int8)
     mesh[0, :] = mesh[-1, :] = mesh[:, 0] = mesh[:, -1] = 0
@@ -266,7 +278,6 @@ def cpu_baseline(steps: int, threads: int):
     fibers = np.zeros((n, n, 2))
     fibers[..., 0] = np.cos(ang)
     fibers[..., 1] = np.sin(ang)
-    used = set_threads(threads)
     sim = OracleSim("tp06", mesh, 0.01, DR, fibers=fibers)
     sim.stims.append(OStim("current", 0, 100, (slice(0, n), slice(0, 11)), duration=1.0))
     sim.initialize()
@@ -277,9 +288,68 @@ def cpu_baseline(steps: int, threads: int):
         sim.u, sim.u_new = sim.u_new, sim.u
     dt = time.perf_counter() - t0
     cells = len(sim.idx) * steps
+    legs = {}
+    # C1 in full: the oracle's run loop (stimulus, sweeps, activation tracker)
+    from oracle.oracle import OTracker
+    m1 = np.ones((102, 102), dtype=np.int8)
+    m1[0, :] = m1[-1, :] = m1[:, 0] = m1[:, -1] = 0
+    c1 = OracleSim("aliev_panfilov", m1, 0.01, 0.25)
+    c1.stims.append(OStim("voltage", 0, 1.0, (slice(0, 102), slice(0, 6))))
+    c1.trackers.append(OTracker("act_time", threshold=0.5))
+    w0 = time.perf_counter()
+    c1.run(10.0)
+    w1 = time.perf_counter() - w0
+    legs["c1"] = dict(steps=c1.step, run_ms=round(w1 * 1e3, 2),
+                      us_per_step=round(w1 / c1.step * 1e6, 2),
+                      gcell=round(10_000 * c1.step / w1 / 1e9, 4))
+    # C3 sample: the 258^3 slab in FP64 (the reference's arithmetic)
+    k3 = 258
+    rs = np.random.RandomState(0)
+    m3 = (1 + (rs.random_sample((k3,) * 3) <= 0.2)).astype(np.int8)
+    m3[0], m3[-1], m3[:, 0], m3[:, -1], m3[:, :, 0], m3[:, :, -1] = 0, 0, 0, 0, 0, 0
+    kk = np.arange(k3)
+    phi = np.deg2rad(-60 + 120 * np.clip(kk - 1, 0, k3 - 3) / (k3 - 3))
+    f3 = np.zeros((k3, k3, k3, 3))
+    f3[..., 0] = np.cos(phi)[None, None, :]
+    f3[..., 1] = np.sin(phi)[None, None, :]
+    c3 = OracleSim("aliev_panfilov", m3, 0.01, 0.25, fibers=f3, stencil="aniso", D_ac=0.111)
+    c3.initialize()
+    del f3
+    c3.one_step()
+    n3 = 3
+    w0 = time.perf_counter()
+    for _ in range(n3):
+        c3.one_step()
+        c3.u, c3.u_new = c3.u_new, c3.u
+    w3 = time.perf_counter() - w0
+    legs["c3"] = dict(steps=n3, gcell=round(len(c3.idx) * n3 / w3 / 1e9, 4),
+                      ms_per_step=round(w3 / n3 * 1e3, 1))
+    del c3
+    # C4 sample: one 256^2 Mitchell-Schaeffer tissue of the ensemble; the
+    # ensemble's rate follows from its total cell-steps
+    from finitewave_amd.workloads import restitution_ensemble
+    _, sims = restitution_ensemble(n=512)
+    total = sum(int(np.count_nonzero(s.tissue.mesh == 1)) * int(math.ceil(s.t_max / 0.01))
+                for s in sims)
+    pick = [s for s in sims if s.tag["side"] == 256][0]
+    c4 = OracleSim("mitchell_schaeffer", pick.tissue.mesh, 0.01, 0.25)
+    c4.initialize()
+    c4.one_step()
+    n4 = 400
+    w0 = time.perf_counter()
+    for _ in range(n4):
+        c4.one_step()
+        c4.u, c4.u_new = c4.u_new, c4.u
+    w4 = time.perf_counter() - w0
+    rate4 = len(c4.idx) * n4 / w4
+    legs["c4"] = dict(sample="400 steps of one 256^2 tissue", gcell=round(rate4 / 1e9, 4),
+                      projected_wall_s=round(total / rate4, 1),
+                      sims_per_s=round(512 / (total / rate4), 5))
     return dict(value=cells / dt / 1e9, unit="Gcell-steps/s", cores=used, kind="port",
+                cpu=_cpu_model(),
                 sample=f"{steps} FP64 steps of the same TP06 1026x1026 aniso workload, "
-                       f"{dt:.1f} s on {used} threads")
+                       f"{dt:.1f} s on {used} threads",
+                legs=legs)
 
 
 def ensemble_leg(n_tissues: int, steps: int, full: bool):
@@ -339,9 +409,18 @@ def ensemble_leg(n_tissues: int, steps: int, full: bool):
         torch.cuda.synchronize()
         wall = time.perf_counter() - w0 + done * sec
         res = [s.result for s in sims]
+        apds = [r["apd90"] for r in res if r.get("apd90") is not None]
+        cvs = [r["cv_front"] for r in res if r.get("cv_front") is not None]
         out["full"] = dict(wall_s=round(wall, 2), sims_per_s=round(n_tissues / wall, 3),
-                           apd90_found=sum(r.get("apd90") is not None for r in res),
-                           cv_found=sum(r.get("cv") is not None for r in res))
+                           captured=sum(bool(r.get("captured")) for r in res),
+                           loss_of_capture=sum(not r.get("captured") for r in res),
+                           apd90_found=len(apds),
+                           apd90_ms_p10_p50_p90=[round(float(x), 2) for x in
+                                                 np.percentile(apds, [10, 50, 90])] if apds else None,
+                           cv_front_found=len(cvs),
+                           cv_front_p10_p50_p90=[round(float(x), 4) for x in
+                                                 np.percentile(cvs, [10, 50, 90])] if cvs else None,
+                           cv_row_found=sum(r.get("cv") is not None for r in res))
     b.close()
     return out
 
@@ -372,10 +451,12 @@ def _fp64_plain(cls, sim):
     return np.array(act.output), np.array(ap.output)
 
 
-def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False):
+def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False,
+                plain_every: int = 4):
     """The Poisson request stream served with continuous batching (FP32,
     TP06 calcium in FP64), then, for comparison and for the error figures,
-    the same requests one after another on the plain FP64 path."""
+    every `plain_every`-th request again, one after another, on the plain
+    FP64 path (its sims/s is that sample's rate)."""
     import torch
     from finitewave_amd.analysis import apd90
     from finitewave_amd.serving import Request, Server
@@ -406,7 +487,8 @@ def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False):
         t_plain = 0.0
         lat_max, lat_bad, lat_n, apd_err = 0.0, 0, 0, []
         per_req = []
-        for r in reqs:
+        sample = reqs[::plain_every]
+        for r in sample:
             w0 = time.perf_counter()
             act64, trace64 = _fp64_plain(classes[r.model], r.sim)
             torch.cuda.synchronize()
@@ -431,11 +513,27 @@ def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False):
                                 nodes=int(both.sum()),
                                 apd90_err_ms=(abs(a - a64) if a is not None and a64 is not None
                                               else None)))
-        plain_rate = n_req / t_plain
-        out["fp64_plain"] = dict(sims_per_s=round(plain_rate, 4), wall_s=round(t_plain, 2))
-        out["speedup_vs_fp64_plain"] = round(stats["sims_per_s"] / plain_rate, 2)
+        plain_rate = len(sample) / t_plain
+        # the served rate over the same requests, for a like-for-like ratio
+        out["fp64_plain"] = dict(sims_per_s=round(plain_rate, 4), wall_s=round(t_plain, 2),
+                                 requests=len(sample),
+                                 sample=f"every {plain_every}th request of the stream",
+                                 cell_steps_per_s_g=round(sum(
+                                     int(np.count_nonzero(r.sim.tissue.mesh == 1)) * r.sim.n_steps
+                                     for r in sample) / t_plain / 1e9, 3))
+        # the plain path's time for the whole stream, projected from the
+        # sample by cell-steps (its cost is the step kernels')
+        cs_sample = sum(int(np.count_nonzero(r.sim.tissue.mesh == 1)) * r.sim.n_steps
+                        for r in sample)
+        plain_all = t_plain * cell_steps / cs_sample
+        out["fp64_plain"]["projected_stream_wall_s"] = round(plain_all, 1)
+        out["fp64_plain"]["projected_sims_per_s"] = round(n_req / plain_all, 4)
+        out["speedup_vs_fp64_plain"] = round(stats["sims_per_s"] / (n_req / plain_all), 2)
         out["errors_vs_fp64"] = dict(lat_max_ms=lat_max, lat_nodes=lat_n,
                                      lat_over_1dt=lat_bad,
+                                     lat_frac_over_1dt=lat_bad / max(1, lat_n),
+                                     lat_p99_ms_worst_request=max(
+                                         (q["lat_p99_ms"] for q in per_req), default=None),
                                      apd90_max_ms=max(apd_err) if apd_err else None,
                                      apd90_compared=len(apd_err))
         if per_request:
@@ -468,9 +566,11 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--ensemble-tissues", type=int, default=512)
     ap.add_argument("--ensemble-steps", type=int, default=2000)
-    ap.add_argument("--ensemble-full", action="store_true",
-                    help="also run the whole ensemble to completion")
-    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--no-ensemble-full", action="store_true",
+                    help="skip running the whole ensemble to completion")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--plain-every", type=int, default=4,
+                    help="run every k-th served request on the plain FP64 path")
     ap.add_argument("--no-fp64-serving", action="store_true")
     ap.add_argument("--per-request", action="store_true",
                     help="list each served request's errors against the FP64 path")
@@ -494,13 +594,16 @@ def main(argv=None) -> int:
         runs[prec] = gpu_run(prec, args.steps, args.warmup)
         r = runs[prec]
         print(f"# {prec}: {r['ms'] / r['steps'] * 1e3:.2f} us/step, finite={r['finite']}, "
-              f"activated={r['activated']}", file=sys.stderr, flush=True)
+              f"activated={r['activated']}, S1 on steps {r['s1_steps']}, "
+              f"{r['stimulated_timed']} of the timed steps stimulated", file=sys.stderr,
+              flush=True)
     primary = "fp64" if "fp64" in runs else "fp32"
     out = {}
     for prec, r in runs.items():
         sec_per_step = r["ms"] / 1e3 / r["steps"]
         bytes_step = (B_STATE[prec] + B_TRACK) * r["n_active"] + r["n_nodes"]
         achieved = bytes_step / sec_per_step / 1e9
+        stored = (B_STORED[prec] + B_TRACK) * r["n_active"] + r["n_nodes"]
         out[prec] = dict(
             gcell=r["n_active"] / sec_per_step / 1e9,
             ms_per_step=sec_per_step * 1e3,
@@ -508,8 +611,11 @@ def main(argv=None) -> int:
                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                       "traffic": profiled_traffic(prec)[0],
                       "traffic_source": profiled_traffic(prec)[1],
-                      "bytes_per_step": bytes_step},
-            finite=r["finite"], activated=r["activated"])
+                      "bytes_per_step": bytes_step,
+                      "stored_bytes_per_step": stored,
+                      "stored_frac": round(stored / sec_per_step / 1e9 / HBM_PEAK_GBS, 4)},
+            finite=r["finite"], activated=r["activated"],
+            stimulated_timed=r["stimulated_timed"], s1_steps=r["s1_steps"])
     p = out[primary]
     line = {
         "metric": "Gcell-steps/s & %HBM roofline; served sims/s + p50/p99 at LAT err <= 1 dt",
@@ -528,21 +634,25 @@ def main(argv=None) -> int:
                                "dr 0.025 cm (0.25 model units), dt 0.01, S1 planar stimulus, "
                                "activation-time tracker",
                    "nodes": runs[primary]["n_nodes"], "active": runs[primary]["n_active"],
-                   "precision": primary},
+                   "precision": primary,
+                   "timed_steps": [args.warmup, args.warmup + args.steps - 1],
+                   "s1_stimulus_steps": list(runs[primary]["s1_steps"][0]),
+                   "stimulated_timed_steps": runs[primary]["stimulated_timed"]},
         "roofline": p["roofline"],
     }
     if "fp32" in out and primary != "fp32":
         f = out["fp32"]
         line["fp32_mixed"] = {"value": round(f["gcell"], 4), "unit": "Gcell-steps/s",
                               "ms_per_step": round(f["ms_per_step"], 5),
-                              "roofline": f["roofline"]}
+                              "roofline": f["roofline"],
+                              "stimulated_timed_steps": f["stimulated_timed"]}
     if "ensemble" in legs:
         line["ensemble"] = ensemble_leg(args.ensemble_tissues, args.ensemble_steps,
-                                        args.ensemble_full)
+                                        not args.no_ensemble_full)
         print(f"# ensemble: {line['ensemble']}", file=sys.stderr, flush=True)
     if "serving" in legs:
         line["serving"] = serving_leg(args.serve_requests, not args.no_fp64_serving,
-                                      args.per_request)
+                                      args.per_request, args.plain_every)
         print(f"# serving: {line['serving']}", file=sys.stderr, flush=True)
     if "c1" in legs:
         line["c1"] = c1_leg()

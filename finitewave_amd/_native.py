@@ -21,14 +21,17 @@ STENCIL_ISO, STENCIL_ANISO, STENCIL_WEIGHTS, STENCIL_SYM = 0, 1, 2, 3
 PREC_FP64, PREC_FP32 = 0, 1
 FIELD_UNIFORM, FIELD_AXIS, FIELD_FULL = 0, 1, 2
 CODE_CONDUCT, CODE_FROZEN, CODE_PROBE = 1, 2, 4
-MAX_STATE, MAX_PARAMS, MAX_FIELDS = 24, 64, 9
+MAX_STATE, MAX_PARAMS, MAX_FIELDS, MAX_STIMS = 24, 64, 9, 8
+ABI_VERSION = 2
 
 # every symbol include/finitewave_amd.h declares
 EXPORTS = (
     "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64", "fw_u_is_f64",
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
-    "fw_stim_apply", "fw_act_time_update", "fw_lat_update", "fw_spiral_tips", "fw_ecg",
+    "fw_run_graph_prepare",
+    "fw_stim_apply", "fw_plan_set_stims", "fw_stim_apply_step", "fw_act_time_update",
+    "fw_lat_update", "fw_spiral_tips", "fw_ecg",
     "fw_fibrosis",
     "fw_batch_create", "fw_batch_destroy",
     "fw_batch_set_tissue", "fw_batch_set_active", "fw_batch_release", "fw_batch_block_nodes",
@@ -61,13 +64,18 @@ class fw_plan_desc(ctypes.Structure):
 class fw_track_desc(ctypes.Structure):
     _fields_ = [("act", c_void_p), ("act_thr", c_double), ("act_flags", c_void_p),
                 ("n_probe", c_int32), ("probe_idx", c_void_p), ("probe_var", c_void_p),
-                ("probe_buf", c_void_p), ("probe_rows", c_void_p)]
+                ("probe_buf", c_void_p), ("probe_rows", c_void_p), ("stim_end", c_int64)]
 
 
 class fw_stim_desc(ctypes.Structure):
     _fields_ = [("kind", c_int32), ("lo", c_int64 * 3), ("hi", c_int64 * 3),
                 ("mask", c_void_p), ("value", c_double), ("clamp", c_int32),
                 ("u_max", c_double)]
+
+
+class fw_stim_sched(ctypes.Structure):
+    _fields_ = [("stim", fw_stim_desc), ("step_lo", c_int64), ("step_hi", c_int64),
+                ("values", c_void_p)]
 
 
 _lib = None
@@ -100,7 +108,10 @@ def lib():
     L.fw_plan_set_weights_k.argtypes = [vp, i32]
     L.fw_run.argtypes = [vp, vp, vp, POINTER(vp), i64, i32, POINTER(fw_track_desc), vp]
     L.fw_run_graph.argtypes = [vp, vp, vp, POINTER(vp), i64, i32, POINTER(fw_track_desc), vp]
+    L.fw_run_graph_prepare.argtypes = [vp, vp, vp, POINTER(vp), i32, POINTER(fw_track_desc)]
     L.fw_stim_apply.argtypes = [vp, vp, POINTER(fw_stim_desc), vp]
+    L.fw_plan_set_stims.argtypes = [vp, POINTER(fw_stim_sched), i32]
+    L.fw_stim_apply_step.argtypes = [vp, vp, i64, vp]
     L.fw_act_time_update.argtypes = [vp, vp, vp, dbl, i32, vp]
     L.fw_lat_update.argtypes = [vp, vp, vp, vp, i32, dbl, i32, vp, vp]
     L.fw_spiral_tips.argtypes = [vp, vp, vp, dbl, vp, i32, vp, vp]
@@ -110,6 +121,10 @@ def lib():
     for name in EXPORTS:
         if name not in ("fw_last_error",):
             getattr(L, name).restype = ctypes.c_int
+    if L.fw_abi_version() != ABI_VERSION:
+        raise NativeUnavailableError(
+            f"{LIB_PATH} has ABI version {L.fw_abi_version()}, this package needs "
+            f"{ABI_VERSION}; rebuild it")
     _lib = L
     return L
 

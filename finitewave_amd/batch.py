@@ -27,7 +27,7 @@ import numpy as np
 
 from . import _native
 from ._native import check, lib
-from .analysis import apd90, conduction_velocity
+from .analysis import apd90, captured, conduction_velocity, front_velocity
 
 
 class _Shim:
@@ -47,6 +47,7 @@ class BatchSim:
     act_threshold: float | None = None
     act_start: float = 0.0         # model time from which activation is recorded
     apd_threshold: float | None = None
+    apd_start: float = 0.0         # model time from which the trace is analysed (S2)
     cv_row: int | None = None      # row for the conduction-velocity estimate
     tag: object = None
     # filled by the batch
@@ -328,11 +329,17 @@ class TissueBatch:
             res["act_t"] = at
             if sim.cv_row is not None:
                 res["cv"] = conduction_velocity(at, sim.cv_row, self.dr)
+            fv = front_velocity(at, self.dr)
+            res["cv_front"] = float(np.mean(fv)) if fv is not None else None
         if buf["probe"] is not None:
             trace = buf["probe"][:sim.n_steps].cpu().numpy().astype(np.float64)
             res["trace"] = trace
             if sim.apd_threshold is not None:
-                res["apd90"] = apd90(trace, self.dt, sim.apd_threshold)
+                # the S2 run's own trace (RestitutionCurve.ipynb cell 6): from
+                # the first step with t >= apd_start on
+                s0 = int(np.searchsorted(fold[:sim.n_steps], sim.apd_start, side="left"))
+                res["captured"] = captured(trace, sim.apd_threshold, s0)
+                res["apd90"] = apd90(trace, self.dt, sim.apd_threshold, s0)
         sim.result = res
         check(lib().fw_batch_release(self.handle, slot), "fw_batch_release")
         self.free.append(slot)

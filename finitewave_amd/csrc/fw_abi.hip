@@ -80,6 +80,13 @@ void complete_params(int model, Params& P) {
 
 __global__ void k_set_word(int32_t* p, int32_t v) { *p = v; }
 
+// graph replay words: the step the next launch reads, and the end of the
+// steps whose scheduled stimuli the kernels apply
+__global__ void k_set_run_words(int32_t* p, int32_t step0, int32_t stim_end) {
+  p[0] = step0;
+  p[2] = stim_end;
+}
+
 __global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, int64_t n) {
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) dst[q] = (float)src[q];
@@ -260,35 +267,36 @@ int fw_ionic(int32_t model, int32_t precision, void* u_new, const void* u, void*
                       n_indexes, dt, P, as_stream(stream));
 }
 
-int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
-  if (!desc || !out) {
-    set_error("fw_plan_create: null argument");
-    return FW_E_INVALID;
-  }
-  *out = nullptr;
+// The half-step fields a stencil reads: ISO ndim, ANISO 4 (2D) or 9 (3D).
+static int needed_fields(const fw_plan_desc& d) {
+  return d.stencil == FW_STENCIL_WEIGHTS ? 0 : d.stencil == FW_STENCIL_ISO ? d.ndim
+                                                : (d.ndim == 2 ? 4 : 9);
+}
+
+// Validates a plan description and fills the host part of a plan (no device
+// memory): the description, node count, launch geometry and field layout.
+// `who` names the entry point in error messages.
+static int plan_host_init(const fw_plan_desc* desc, fw_plan* p, const char* who) {
   const fw_plan_desc& d = *desc;
   if (d.ndim != 2 && d.ndim != 3) {
-    set_error("fw_plan_create: ndim must be 2 or 3 (got %d)", d.ndim);
+    set_error("%s: ndim must be 2 or 3 (got %d)", who, d.ndim);
     return FW_E_INVALID;
   }
   if (d.model < 0 || d.model >= FW_N_MODELS) {
-    set_error("fw_plan_create: unknown model %d", d.model);
+    set_error("%s: unknown model %d", who, d.model);
     return FW_E_INVALID;
   }
   if (d.precision != FW_PREC_FP64 && d.precision != FW_PREC_FP32) {
-    set_error("fw_plan_create: bad precision %d", d.precision);
+    set_error("%s: bad precision %d", who, d.precision);
     return FW_E_INVALID;
   }
   if (d.stencil < FW_STENCIL_ISO || d.stencil > FW_STENCIL_WEIGHTS) {
-    set_error("fw_plan_create: bad stencil %d", d.stencil);
+    set_error("%s: bad stencil %d", who, d.stencil);
     return FW_E_INVALID;
   }
-  const int need_fields = d.stencil == FW_STENCIL_WEIGHTS ? 0
-                          : d.stencil == FW_STENCIL_ISO  ? d.ndim
-                          : (d.ndim == 2 ? 4 : 9);
+  const int need_fields = needed_fields(d);
   if (d.n_fields < need_fields || d.n_fields > FW_MAX_FIELDS) {
-    set_error("fw_plan_create: stencil needs %d half-step fields (got %d)", need_fields,
-              d.n_fields);
+    set_error("%s: stencil needs %d half-step fields (got %d)", who, need_fields, d.n_fields);
     return FW_E_INVALID;
   }
   for (int q = 0; q < need_fields; ++q) {
@@ -296,44 +304,39 @@ int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
     if (f.mode < FW_FIELD_UNIFORM || f.mode > FW_FIELD_FULL ||
         (f.mode != FW_FIELD_UNIFORM && !f.data) ||
         (f.mode == FW_FIELD_AXIS && (f.axis < 0 || f.axis >= d.ndim))) {
-      set_error("fw_plan_create: field %d is malformed", q);
+      set_error("%s: field %d is malformed", who, q);
       return FW_E_INVALID;
     }
   }
   if (d.stencil == FW_STENCIL_WEIGHTS && !d.weights) {
-    set_error("fw_plan_create: FW_STENCIL_WEIGHTS needs a weights array");
+    set_error("%s: FW_STENCIL_WEIGHTS needs a weights array", who);
     return FW_E_INVALID;
   }
   if (d.n_params != model_n_params(d.model)) {
-    set_error("fw_plan_create: model %d takes %d parameters (got %d)", d.model,
+    set_error("%s: model %d takes %d parameters (got %d)", who, d.model,
               model_n_params(d.model), d.n_params);
     return FW_E_INVALID;
   }
   if (!d.code) {
-    set_error("fw_plan_create: code array is null");
+    set_error("%s: code array is null", who);
     return FW_E_INVALID;
   }
   if (!(d.dt > 0) || !(d.dr2 > 0)) {
-    set_error("fw_plan_create: dt and dr2 must be positive");
+    set_error("%s: dt and dr2 must be positive", who);
     return FW_E_INVALID;
   }
   int64_t N = 1;
   for (int a = 0; a < d.ndim; ++a) {
     if (d.shape[a] < 3) {
-      set_error("fw_plan_create: every axis needs at least 3 nodes (axis %d has %lld)", a,
+      set_error("%s: every axis needs at least 3 nodes (axis %d has %lld)", who, a,
                 (long long)d.shape[a]);
       return FW_E_INVALID;
     }
     N *= d.shape[a];
   }
   if (N > ((int64_t)1 << 40)) {
-    set_error("fw_plan_create: grid too large");
+    set_error("%s: grid too large", who);
     return FW_E_INVALID;
-  }
-  fw_plan* p = new (std::nothrow) fw_plan();
-  if (!p) {
-    set_error("fw_plan_create: out of host memory");
-    return FW_E_NOMEM;
   }
   p->desc = d;
   if (d.ndim == 2) p->desc.shape[2] = 1;
@@ -344,7 +347,51 @@ int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
   p->K = d.ndim == 2 ? 9 : 19;
   p->graph = nullptr;
   p->graph_steps = 0;
-  hipError_t e = hipMalloc(&p->step_words, 2 * sizeof(int32_t));
+  // the field layout: every field uniform (kFieldsUniform), or in 3D every
+  // field uniform or tabulated along the last axis (kFieldsK, layered
+  // fibres); such plans get their step kernel without the per-field mode
+  // dispatch
+  p->field_layout = fw::kFieldsGeneral;
+  if (d.stencil != FW_STENCIL_WEIGHTS) {
+    bool uni = true, along_k = d.ndim == 3;
+    for (int q = 0; q < need_fields; ++q) {
+      const fw_field& f = d.fields[q];
+      if (f.mode != FW_FIELD_UNIFORM) uni = false;
+      if (!(f.mode == FW_FIELD_UNIFORM || (f.mode == FW_FIELD_AXIS && f.axis == 2)))
+        along_k = false;
+    }
+    if (uni) p->field_layout = fw::kFieldsUniform;
+    else if (along_k) p->field_layout = fw::kFieldsK;
+  }
+  return FW_OK;
+}
+
+// Whether a plan's step kernel reads float copies of its tabulated fields
+// (FP32 plans whose potential is float)
+static bool wants_float_tables(const fw_plan_desc& d) {
+  return d.precision == FW_PREC_FP32 && !u_is_f64(d.model, d.precision);
+}
+
+int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
+  if (!desc || !out) {
+    set_error("fw_plan_create: null argument");
+    return FW_E_INVALID;
+  }
+  *out = nullptr;
+  const fw_plan_desc& d = *desc;
+  fw_plan* p = new (std::nothrow) fw_plan();
+  if (!p) {
+    set_error("fw_plan_create: out of host memory");
+    return FW_E_NOMEM;
+  }
+  int rc = plan_host_init(desc, p, "fw_plan_create");
+  if (rc) {
+    delete p;
+    return rc;
+  }
+  const int need_fields = needed_fields(d);
+  const int64_t N = p->N;
+  hipError_t e = hipMalloc(&p->step_words, 4 * sizeof(int32_t));
   if (e != hipSuccess) {
     set_error("fw_plan_create: hipMalloc failed: %s", hipGetErrorString(e));
     delete p;
@@ -352,7 +399,7 @@ int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
   }
   // a kernel computing the diffusion in float reads float copies of the
   // tables: 4-byte loads and no conversion per node
-  if (d.precision == FW_PREC_FP32 && !u_is_f64(d.model, d.precision)) {
+  if (wants_float_tables(d)) {
     bool synced = false;
     for (int q = 0; q < need_fields; ++q) {
       const fw_field& f = d.fields[q];
@@ -382,41 +429,25 @@ int fw_plan_create(const fw_plan_desc* desc, fw_plan** out) {
       }
     }
   }
-  // the field layout: every field uniform (kFieldsUniform), or in 3D every
-  // field uniform or tabulated along the last axis (kFieldsK, layered
-  // fibres); such plans get their step kernel without the per-field mode
-  // dispatch
-  p->field_layout = fw::kFieldsGeneral;
-  if (d.stencil != FW_STENCIL_WEIGHTS) {
-    bool uni = true, along_k = d.ndim == 3;
-    for (int q = 0; q < need_fields; ++q) {
+  if (p->field_layout == fw::kFieldsK) {
+    // each field as a table along k, uniform ones filled with their value
+    const int64_t nk = d.shape[2];
+    (void)hipDeviceSynchronize();  // the tables' uploads on any stream
+    for (int q = 0; q < need_fields && e == hipSuccess; ++q) {
       const fw_field& f = d.fields[q];
-      if (f.mode != FW_FIELD_UNIFORM) uni = false;
-      if (!(f.mode == FW_FIELD_UNIFORM || (f.mode == FW_FIELD_AXIS && f.axis == 2)))
-        along_k = false;
+      e = hipMalloc(&p->ktab[q], nk * sizeof(double));
+      if (e == hipSuccess) e = hipMalloc(&p->fktab[q], nk * sizeof(float));
+      if (e == hipSuccess)
+        hipLaunchKernelGGL(k_fill_ktab, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, 0,
+                           f.mode == FW_FIELD_AXIS ? f.data : nullptr, f.value, p->ktab[q],
+                           p->fktab[q], nk);
+      if (e == hipSuccess) e = hipGetLastError();
     }
-    if (uni) {
-      p->field_layout = fw::kFieldsUniform;
-    } else if (along_k) {
-      const int64_t nk = d.shape[2];
-      (void)hipDeviceSynchronize();  // the tables' uploads on any stream
-      for (int q = 0; q < need_fields && e == hipSuccess; ++q) {
-        const fw_field& f = d.fields[q];
-        e = hipMalloc(&p->ktab[q], nk * sizeof(double));
-        if (e == hipSuccess) e = hipMalloc(&p->fktab[q], nk * sizeof(float));
-        if (e == hipSuccess)
-          hipLaunchKernelGGL(k_fill_ktab, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, 0,
-                             f.mode == FW_FIELD_AXIS ? f.data : nullptr, f.value, p->ktab[q],
-                             p->fktab[q], nk);
-        if (e == hipSuccess) e = hipGetLastError();
-      }
-      if (e == hipSuccess) e = hipDeviceSynchronize();
-      if (e != hipSuccess) {
-        set_error("fw_plan_create: field tables along k: %s", hipGetErrorString(e));
-        fw_plan_destroy(p);
-        return FW_E_NOMEM;
-      }
-      p->field_layout = fw::kFieldsK;
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+      set_error("fw_plan_create: field tables along k: %s", hipGetErrorString(e));
+      fw_plan_destroy(p);
+      return FW_E_NOMEM;
     }
   }
   *out = p;
@@ -449,6 +480,11 @@ int fw_plan_destroy(fw_plan* plan) {
   return FW_OK;
 }
 
+// a step number as the kernels hold it
+static int32_t clamp_step(int64_t s) {
+  return s < INT32_MIN ? INT32_MIN : (s > INT32_MAX ? INT32_MAX : (int32_t)s);
+}
+
 static int check_run_args(fw_plan* plan, void* u, void* u_new, void* const* state,
                           int32_t n_steps, const fw_track_desc* track) {
   if (!plan || !u || !u_new || !state || n_steps < 0) {
@@ -469,17 +505,19 @@ static int check_run_args(fw_plan* plan, void* u, void* u_new, void* const* stat
   return FW_OK;
 }
 
-int fw_run(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
-           int32_t n_steps, const fw_track_desc* track, void* stream) {
-  int rc = check_run_args(plan, u, u_new, state, n_steps, track);
-  if (rc) return rc;
+// n_steps plain launches of the step kernel (arguments already checked)
+static int run_plain(const fw_plan* plan, void* u, void* u_new, void* const* state,
+                     int64_t step0, int32_t n_steps, const fw_track_desc* track, void* stream) {
+  int rc = FW_OK;
   StepLauncher L = step_launcher(plan->desc.precision, plan->desc.model);
   const StatePtrs st = make_state(state, model_n_state(plan->desc.model));
   void* a = u;
   void* b = u_new;
+  const int32_t stim_end = track ? clamp_step(track->stim_end) : 0;
   for (int32_t s = 0; s < n_steps; ++s) {
     TrackArgs tr{};
     tr.step = (int32_t)(step0 + s);
+    tr.stim_end = stim_end;
     if (track) {
       tr.act = track->act;
       tr.act_thr = track->act_thr;
@@ -499,6 +537,13 @@ int fw_run(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step
     b = t;
   }
   return FW_OK;
+}
+
+int fw_run(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
+           int32_t n_steps, const fw_track_desc* track, void* stream) {
+  int rc = check_run_args(plan, u, u_new, state, n_steps, track);
+  if (rc) return rc;
+  return run_plain(plan, u, u_new, state, step0, n_steps, track, stream);
 }
 
 // Pairs of steps per launch of the long graph: a graph launch costs several
@@ -527,6 +572,7 @@ static int capture_pairs(fw_plan* plan, void* u, void* u_new, const StatePtrs& s
     tr.act_thr = act_thr;
     tr.act_on = act_on;
     tr.probe_row = -1;
+    tr.stim_end_in = plan->step_words + 2;
     for (int32_t q = 0; q < pairs && !rc; ++q) {
       tr.step_in = plan->step_words;
       tr.step_out = plan->step_words + 1;
@@ -549,8 +595,10 @@ static int capture_pairs(fw_plan* plan, void* u, void* u_new, const StatePtrs& s
   return FW_OK;
 }
 
-int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
-                 int32_t n_steps, const fw_track_desc* track, void* stream) {
+// The graphs fw_run_graph replays for these buffers, tracker and schedule:
+// captured now unless the cached ones match.  *act_on_out: the gate.
+static int ensure_graphs(fw_plan* plan, void* u, void* u_new, void* const* state,
+                         int32_t n_steps, const fw_track_desc* track, int32_t* act_on_out) {
   int rc = check_run_args(plan, u, u_new, state, n_steps, track);
   if (rc) return rc;
   if (track && track->n_probe > 0) {
@@ -569,13 +617,14 @@ int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_
           return FW_E_UNSUPPORTED;
         }
   }
+  *act_on_out = act_on;
   const double act_thr = track ? track->act_thr : 0.0;
   const int ns = model_n_state(plan->desc.model);
   const int32_t pairs = n_steps / 2;
-  hipStream_t s = as_stream(stream);
   // cached graphs still valid for these buffers?
   bool reuse = plan->graph && plan->graph_u == u && plan->graph_un == u_new &&
-               plan->graph_act == act && plan->graph_act_on == act_on;
+               plan->graph_act == act && plan->graph_act_on == act_on &&
+               (!act || plan->graph_act_thr == act_thr);
   for (int k = 0; reuse && k < ns; ++k) reuse = plan->graph_state[k] == state[k];
   const StatePtrs st = make_state(state, ns);
   if (!reuse && pairs > 0) {
@@ -589,6 +638,7 @@ int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_
     plan->graph_un = u_new;
     plan->graph_act = act;
     plan->graph_act_on = act_on;
+    plan->graph_act_thr = act_thr;
     for (int k = 0; k < FW_MAX_STATE; ++k) plan->graph_state[k] = k < ns ? state[k] : nullptr;
   }
   if (pairs >= kGraphLongPairs && !plan->graph_long) {
@@ -596,8 +646,25 @@ int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_
                        &plan->graph_long);
     if (rc) return rc;
   }
+  return FW_OK;
+}
+
+int fw_run_graph_prepare(fw_plan* plan, void* u, void* u_new, void* const* state,
+                         int32_t n_steps, const fw_track_desc* track) {
+  int32_t act_on = 0;
+  return ensure_graphs(plan, u, u_new, state, n_steps, track, &act_on);
+}
+
+int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
+                 int32_t n_steps, const fw_track_desc* track, void* stream) {
+  int32_t act_on = 0;
+  int rc = ensure_graphs(plan, u, u_new, state, n_steps, track, &act_on);
+  if (rc) return rc;
+  const int32_t pairs = n_steps / 2;
+  hipStream_t s = as_stream(stream);
   if (pairs > 0) {
-    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, plan->step_words, (int32_t)step0);
+    hipLaunchKernelGGL(k_set_run_words, dim3(1), dim3(1), 0, s, plan->step_words,
+                       (int32_t)step0, track ? clamp_step(track->stim_end) : 0);
     int32_t q = 0;
     for (; q + kGraphLongPairs <= pairs; q += kGraphLongPairs) {
       hipError_t e = hipGraphLaunch(plan->graph_long, s);
@@ -635,6 +702,66 @@ int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim, void* stream
   return launch_stim(u_is_f64(plan->desc.model, plan->desc.precision), u, plan->desc.code, stim,
                      plan->desc.shape,
                      as_stream(stream));
+}
+
+int fw_plan_set_stims(fw_plan* plan, const fw_stim_sched* stims, int32_t n) {
+  if (!plan || n < 0 || n > FW_MAX_STIMS || (n > 0 && !stims)) {
+    set_error("fw_plan_set_stims: bad arguments (n = %d, at most %d)", n, FW_MAX_STIMS);
+    return FW_E_INVALID;
+  }
+  const int nd = plan->desc.ndim;
+  for (int32_t q = 0; q < n; ++q) {
+    const fw_stim_sched& ss = stims[q];
+    if (ss.stim.kind != 0 && ss.stim.kind != 1) {
+      set_error("fw_plan_set_stims: stimulus %d: kind %d", q, ss.stim.kind);
+      return FW_E_INVALID;
+    }
+    if (ss.step_lo < 0 || ss.step_hi < ss.step_lo || ss.step_hi > INT32_MAX) {
+      set_error("fw_plan_set_stims: stimulus %d: steps [%lld, %lld]", q,
+                (long long)ss.step_lo, (long long)ss.step_hi);
+      return FW_E_INVALID;
+    }
+    for (int a = 0; a < 3; ++a) {
+      const int64_t n_a = a < nd ? plan->desc.shape[a] : 1;
+      if (ss.stim.lo[a] < 0 || ss.stim.hi[a] > n_a || ss.stim.lo[a] > ss.stim.hi[a]) {
+        set_error("fw_plan_set_stims: stimulus %d: axis %d box [%lld, %lld) outside [0, %lld)",
+                  q, a, (long long)ss.stim.lo[a], (long long)ss.stim.hi[a], (long long)n_a);
+        return FW_E_INVALID;
+      }
+    }
+    if (ss.values && ss.stim.kind != 0) {
+      set_error("fw_plan_set_stims: stimulus %d: a value list needs a voltage stimulus", q);
+      return FW_E_INVALID;
+    }
+  }
+  const bool same = n == plan->n_stims &&
+                    (n == 0 || memcmp(plan->stims, stims, n * sizeof(fw_stim_sched)) == 0);
+  if (!same) {
+    // captured graphs hold the old schedule in their kernel arguments
+    if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
+    if (plan->graph_long) (void)hipGraphExecDestroy(plan->graph_long);
+    plan->graph = nullptr;
+    plan->graph_long = nullptr;
+    plan->n_stims = n;
+    if (n > 0) memcpy(plan->stims, stims, n * sizeof(fw_stim_sched));
+  }
+  return FW_OK;
+}
+
+int fw_stim_apply_step(fw_plan* plan, void* u, int64_t step, void* stream) {
+  if (!plan || !u) {
+    set_error("fw_stim_apply_step: bad arguments");
+    return FW_E_INVALID;
+  }
+  for (int32_t q = 0; q < plan->n_stims; ++q) {
+    const fw_stim_sched& ss = plan->stims[q];
+    if (step < ss.step_lo || step > ss.step_hi) continue;
+    const int rc = launch_stim(u_is_f64(plan->desc.model, plan->desc.precision), u,
+                               plan->desc.code, &ss.stim, plan->desc.shape, as_stream(stream),
+                               ss.values ? ss.values + (step - ss.step_lo) : nullptr);
+    if (rc) return rc;
+  }
+  return FW_OK;
 }
 
 int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr, int32_t step,

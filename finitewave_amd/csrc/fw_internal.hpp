@@ -14,7 +14,7 @@ struct fw_plan {
   uint32_t per_xcd;  // logical blocks per XCD
   uint32_t grid;     // launched blocks = per_xcd * 8
   int32_t K;         // weights per node for FW_STENCIL_WEIGHTS
-  int32_t* step_words;  // device int32[2] for graph replay
+  int32_t* step_words;  // device int32[4]: graph step words [0..1], stim_end [2]
   hipGraphExec_t graph;       // two steps (u -> u_new -> u)
   hipGraphExec_t graph_long;  // kGraphLongPairs such pairs, for long runs
   int32_t graph_steps;
@@ -23,6 +23,11 @@ struct fw_plan {
   void* graph_state[FW_MAX_STATE];
   int32_t graph_act_on;
   int32_t* graph_act;
+  double graph_act_thr;
+  // the stimulus schedule (fw_plan_set_stims), copied into each step's
+  // kernel arguments
+  int32_t n_stims;
+  fw_stim_sched stims[FW_MAX_STIMS];
   // FP32 plans with a float potential: float copies of the tabulated
   // half-step fields (made once by fw_plan_create)
   float* ftab[FW_MAX_FIELDS];
@@ -53,6 +58,10 @@ struct TrackArgs {
   const int32_t* probe_var;
   double* probe_buf;
   int32_t probe_row;
+  // scheduled stimuli of step + 1 are applied while step + 1 < stim_end,
+  // read from *stim_end_in when that is set (graph replay)
+  int32_t stim_end;
+  const int32_t* stim_end_in;
 };
 
 typedef int (*StepLauncher)(const fw_plan* plan, const void* u, void* u_new,
@@ -89,8 +98,10 @@ int launch_diffusion(int ndim, int K, double* u_new, const double* u,
 int launch_ionic(int model, int precision, void* u_new, const void* u,
                  const StatePtrs& st, const int64_t* idx, int64_t n,
                  double dt, const Params& P, hipStream_t s);
+// value_dev: a device value that replaces stim->value of a voltage stimulus
 int launch_stim(int u_f64, void* u, const uint8_t* code,
-                const fw_stim_desc* stim, const int64_t* shape, hipStream_t s);
+                const fw_stim_desc* stim, const int64_t* shape, hipStream_t s,
+                const double* value_dev = nullptr);
 int launch_act(int u_f64, int32_t* act, const void* u, int64_t n,
                double thr, int32_t step, hipStream_t s);
 

@@ -123,6 +123,20 @@ struct MinorQ {
   double nq[2], pq[2], dc[2][2];
 };
 
+// One scheduled stimulus (fw_stim_sched): applied to the pre-step potential
+// of the steps [s_lo, s_hi] at conducting nodes inside the box [lo, hi) and
+// the mask; kind 0 sets `value` (or vals[s - s_lo]), kind 1 adds it and
+// clamps at u_max when `clamp`
+constexpr int kMaxStims = 8;
+struct StimRec {
+  int32_t lo[3], hi[3];
+  int32_t kind, clamp;
+  int32_t s_lo, s_hi;
+  const uint8_t* mask;
+  const double* vals;
+  double value, u_max;
+};
+
 // R: arithmetic type of the ionic update; U: storage type of u / u_new
 // (UType<MODEL, R>, double for the millivolt-scale models on the FP32 path)
 template <typename R, typename U = R>
@@ -158,6 +172,13 @@ struct KArgs {
   // kFieldsUniform, FP64, 2D: the final weights (epilogue applied) of a node
   // whose 3x3 neighbourhood all conducts -- the same for every such node
   double wfull[9];
+  // the stimulus schedule: the kernel of step s applies the stimuli of step
+  // s + 1 to the u_new it writes while s + 1 < stim_end (*stim_end_in in
+  // graph replay)
+  int32_t n_stim;
+  int32_t stim_end;
+  const int32_t* stim_end_in;
+  StimRec stim[kMaxStims];
 };
 
 // (i, j, k) of flat node index c in a C-order grid (k = 0 in 2D)
@@ -877,6 +898,41 @@ __device__ __forceinline__ void probe_node(const A& a, int64_t c, uint8_t code, 
   }
 }
 
+// Whether any scheduled stimulus applies at step s (uniform across a launch)
+template <typename A>
+__device__ __forceinline__ bool stim_due(const A& a, int32_t s) {
+  if (a.n_stim == 0) return false;
+  const int32_t end = a.stim_end_in ? *a.stim_end_in : a.stim_end;
+  if (s >= end) return false;
+  bool due = false;
+  for (int q = 0; q < a.n_stim; ++q) due |= s >= a.stim[q].s_lo && s <= a.stim[q].s_hi;
+  return due;
+}
+
+// The scheduled stimuli of step s applied, in schedule order, to the value v
+// a conducting node (i, j, k) = c holds before step s: the operations of
+// Stim*.stimulate on that node (u = V; u += dt*I, then u_max), in the
+// potential's storage type as fw_stim_apply does them
+template <typename U, typename A>
+__device__ __forceinline__ U stim_node(const A& a, int32_t s, int64_t i, int64_t j, int64_t k,
+                                       int64_t c, U v) {
+  for (int q = 0; q < a.n_stim; ++q) {
+    const StimRec& r = a.stim[q];
+    if (s < r.s_lo || s > r.s_hi) continue;
+    if (i < r.lo[0] || i >= r.hi[0] || j < r.lo[1] || j >= r.hi[1] || k < r.lo[2] ||
+        k >= r.hi[2])
+      continue;
+    if (r.mask && r.mask[c] == 0) continue;
+    if (r.kind == 0) {
+      v = (U)(r.vals ? r.vals[s - r.s_lo] : r.value);
+    } else {
+      v = v + (U)r.value;
+      if (r.clamp && v > (U)r.u_max) v = (U)r.u_max;
+    }
+  }
+  return v;
+}
+
 // One node of the fused step: activation time, the update of an interior
 // updated node, probes.  FL: the plan's field layout (KArgs::field_layout).
 // The uniform and along-k instances read the field values without the
@@ -899,19 +955,34 @@ __device__ __forceinline__ void step_node(const KArgs<R, UType<MODEL, R>>& a, in
     bool interior = i > 0 && i < a.ni - 1 && j > 0 && j < a.nj - 1;
     if constexpr (NDIM == 3) interior = interior && k > 0 && k < a.nk - 1;
     if (interior) {
+      U v;
       if constexpr (NDIM == 2 && PREC != 0 && ST != 2) {
         // FP32 flux form: the neighbourhood is loaded before the ionic
         // update, as in the batched step
         const Hood2<U> q = load_hood2<ST>(a, c, uc);
-        a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
+        v = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
           return flux2d<ST, FL == kFieldsUniform>(a, i, j, c, uc, q);
         });
       } else {
-        a.u_new[c] = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
+        v = node_update<MODEL, R>(uc, a.st, c, a.dt, a.P, [&] {
           return diffuse<NDIM, ST, R, U, PREC == 0, FL>(a, i, j, k, c, uc);
         });
       }
+      // the next step's stimuli, on the potential this step hands it; the
+      // coordinates are recomputed so they need not live across the update
+      if (stim_due(a, step + 1)) {
+        int64_t si, sj, sk;
+        node_coords(a, c, si, sj, sk);
+        v = stim_node<U>(a, step + 1, si, sj, sk, c, v);
+      }
+      a.u_new[c] = v;
     }
+  } else if ((code & 3) == 3 && stim_due(a, step + 1)) {
+    // a frozen node is never updated; the next step's stimuli still apply
+    // to the value its u_new buffer holds (the next step's potential)
+    int64_t i, j, k;
+    node_coords(a, c, i, j, k);
+    a.u_new[c] = stim_node<U>(a, step + 1, i, j, k, c, a.u_new[c]);
   }
   probe_node<R, MODEL, PREC>(a, c, code, (double)uc);
 }
@@ -1134,7 +1205,8 @@ __global__ __launch_bounds__(kBlock) void k_stim(R* __restrict__ u,
                                                  int64_t lo0, int64_t lo1, int64_t lo2,
                                                  int64_t b0, int64_t b1, int64_t b2,
                                                  int64_t nj, int64_t nk, int kind,
-                                                 double value, int clamp, double u_max) {
+                                                 double value, const double* value_dev,
+                                                 int clamp, double u_max) {
   const int64_t n = b0 * b1 * b2;
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (q >= n) return;
@@ -1143,7 +1215,7 @@ __global__ __launch_bounds__(kBlock) void k_stim(R* __restrict__ u,
   if ((code[c] & 1) == 0) return;
   if (mask && mask[c] == 0) return;
   if (kind == 0) {
-    u[c] = (R)value;
+    u[c] = (R)(value_dev ? *value_dev : value);
   } else {
     R v = u[c] + (R)value;
     if (clamp && v > (R)u_max) v = (R)u_max;

@@ -102,7 +102,7 @@ int launch_ionic(int model, int precision, void* u_new, const void* u,
 }
 
 int launch_stim(int u_f64, void* u, const uint8_t* code, const fw_stim_desc* st,
-                const int64_t* shape, hipStream_t s) {
+                const int64_t* shape, hipStream_t s, const double* value_dev) {
   int64_t lo[3], ext[3];
   for (int a = 0; a < 3; ++a) {
     int64_t l = st->lo[a] < 0 ? 0 : st->lo[a];
@@ -116,11 +116,11 @@ int launch_stim(int u_f64, void* u, const uint8_t* code, const fw_stim_desc* st,
   if (u_f64)
     hipLaunchKernelGGL((k_stim<double>), g, b, 0, s, static_cast<double*>(u), code, st->mask,
                        lo[0], lo[1], lo[2], ext[0], ext[1], ext[2], shape[1], shape[2],
-                       st->kind, st->value, st->clamp, st->u_max);
+                       st->kind, st->value, value_dev, st->clamp, st->u_max);
   else
     hipLaunchKernelGGL((k_stim<float>), g, b, 0, s, static_cast<float*>(u), code, st->mask,
                        lo[0], lo[1], lo[2], ext[0], ext[1], ext[2], shape[1], shape[2],
-                       st->kind, st->value, st->clamp, st->u_max);
+                       st->kind, st->value, value_dev, st->clamp, st->u_max);
   return check_launch("stim");
 }
 

@@ -130,6 +130,30 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
   a.probe_var = tr.probe_var;
   a.probe_buf = tr.probe_buf;
   a.probe_row = tr.probe_buf ? tr.probe_row : -1;
+  a.n_stim = 0;
+  a.stim_end = tr.stim_end;
+  a.stim_end_in = tr.stim_end_in;
+}
+
+// the plan's stimulus schedule into the step kernel's arguments
+static void fill_stims(const fw_plan* plan, KArgs<Real, UReal>& a) {
+  a.n_stim = plan->n_stims;
+  for (int q = 0; q < plan->n_stims; ++q) {
+    const fw_stim_sched& ss = plan->stims[q];
+    StimRec& r = a.stim[q];
+    for (int x = 0; x < 3; ++x) {
+      r.lo[x] = (int32_t)ss.stim.lo[x];
+      r.hi[x] = (int32_t)ss.stim.hi[x];
+    }
+    r.kind = ss.stim.kind;
+    r.clamp = ss.stim.clamp;
+    r.s_lo = (int32_t)ss.step_lo;
+    r.s_hi = (int32_t)ss.step_hi;
+    r.mask = ss.stim.mask;
+    r.vals = ss.values;
+    r.value = ss.stim.value;
+    r.u_max = ss.stim.u_max;
+  }
 }
 
 int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
@@ -137,6 +161,7 @@ int FW_LAUNCHER(const fw_plan* plan, const void* u, void* u_new,
   const fw_plan_desc& d = plan->desc;
   KArgs<Real, UReal> a;
   fill_args(plan, u, u_new, st, tr, a);
+  fill_stims(plan, a);
   const int st_kind = d.stencil;
   if (d.ndim == 2) {
     if (st_kind == 0) launch<2, 0>(plan, a, s);

@@ -153,6 +153,8 @@ class Engine:
         self.plan = None
         self.swapped = False
         self.act = {}  # tracker -> int32 device tensor
+        self.stim_sched = []  # fw_stim_sched entries of the current run
+        self._stim_keep = []
         self.build()
 
     # -- geometry / plan ---------------------------------------------------
@@ -197,6 +199,44 @@ class Engine:
         self.state_names = model._state_call_order()
         self.state_f64 = [_native.state_is_f64(self.model_id, self.precision, k)
                           for k in range(len(self.state_names))]
+        if self.stim_sched:
+            self._push_stims()
+
+    # -- stimulus schedule -------------------------------------------------------
+    def set_stim_schedule(self, entries) -> None:
+        """entries: [(fw_stim_desc, device value list or None, first step,
+        last step)] in sequence order (fw_plan_set_stims)."""
+        if len(entries) > _native.MAX_STIMS:
+            raise ValueError(f"at most {_native.MAX_STIMS} scheduled stimuli")
+        sched = []
+        keep = []
+        for desc, vals, lo, hi in entries:
+            e = _native.fw_stim_sched()
+            e.stim = desc
+            e.step_lo = int(lo)
+            e.step_hi = int(hi)
+            e.values = vals.data_ptr() if vals is not None else 0
+            sched.append(e)
+            keep.append(vals)
+        self.stim_sched = sched
+        self._stim_keep = keep
+        self._push_stims()
+
+    def clear_stim_schedule(self) -> None:
+        self.stim_sched = []
+        self._stim_keep = []
+        self._push_stims()
+
+    def _push_stims(self) -> None:
+        n = len(self.stim_sched)
+        arr = (_native.fw_stim_sched * max(1, n))(*self.stim_sched)
+        check(lib().fw_plan_set_stims(self.plan.handle, arr, n), "fw_plan_set_stims")
+
+    def apply_stims_at(self, step: int) -> None:
+        """The scheduled stimuli of `step` on the current potential."""
+        if self.stim_sched:
+            check(lib().fw_stim_apply_step(self.plan.handle, self.u_dev().data_ptr(),
+                                           int(step), _stream()), "fw_stim_apply_step")
 
     def set_probe_bits(self, flat_cells) -> None:
         self.probe_cells = list(flat_cells)
@@ -234,13 +274,21 @@ class Engine:
 
     # -- stepping ------------------------------------------------------------
     def run_steps(self, step0: int, n: int, act_tracker=None, act_flags=None,
-                  probes=None, probe_rows=None, use_graph: bool = True) -> None:
-        """n fused steps starting at global step step0."""
+                  probes=None, probe_rows=None, use_graph: bool = True,
+                  stim_first: bool = True) -> None:
+        """n fused steps starting at global step step0.  The scheduled
+        stimuli of step0 are applied first unless stim_first is False (the
+        caller applied them); those of the later steps are applied by the
+        step kernels."""
         if n <= 0:
             return
         L = lib()
         track = _native.fw_track_desc()
         keep = []
+        if self.stim_sched:
+            if stim_first:
+                self.apply_stims_at(step0)
+            track.stim_end = step0 + n
         if act_tracker is not None:
             act = self.act[act_tracker]
             track.act = act.data_ptr()
@@ -281,6 +329,24 @@ class Engine:
                 self._plain(track, s0, n, act_flags, s0 - step0, stream)
             return
         self._plain(track, s0, n, act_flags, 0, stream)
+
+    def prepare_graphs(self, n: int, act_tracker=None, act_on: bool = True) -> None:
+        """Captures the graphs a graph-replayed run_steps of n steps would
+        use (fw_run_graph_prepare), launching nothing; run_steps then only
+        replays them."""
+        pairs = n // 2
+        if pairs <= 0:
+            return
+        track = _native.fw_track_desc()
+        track.stim_end = 0
+        flags = np.full(2 * pairs, 1 if act_on else 0, dtype=np.uint8)
+        if act_tracker is not None:
+            track.act = self.act[act_tracker].data_ptr()
+            track.act_thr = float(act_tracker.threshold)
+            track.act_flags = flags.ctypes.data
+        check(lib().fw_run_graph_prepare(self.plan.handle, self.A.data_ptr(), self.B.data_ptr(),
+                                         self._state_ptrs, 2 * pairs, byref(track)),
+              "fw_run_graph_prepare")
 
     def _plain(self, track, step0, n, act_flags, offset, stream) -> None:
         sub = _native.fw_track_desc.from_buffer_copy(track)

@@ -3,7 +3,9 @@
 Mirrors ``core/model/cardiac_model.py``.  Per step: stimuli -> diffusion ->
 ionic -> trackers -> ``t += dt`` -> ``step += 1`` -> swap -> commands ->
 saver -> termination.  Diffusion, ionic update and the activation-time /
-point-probe trackers are one fused GPU kernel; stretches of steps with no
+point-probe trackers are one fused GPU kernel; stimuli are resolved into a
+device schedule that the same kernel applies (each step's kernel stimulates
+the potential it hands to the next step); stretches of steps with no
 host-side event run back to back inside the library (optionally replayed
 from a HIP graph); steps with a command, a state save or a host-only tracker
 synchronise the host numpy view of the state first.
@@ -262,6 +264,8 @@ class CardiacModel:
             chunk_row_owners.clear()
 
         dt = self.dt
+        # stimuli on the device schedule: their steps need no host event
+        sched = self._schedule_stims(eng, stims, iters)
         for _ in range(iters):
             t, step = self.t, self.step
             if len(chunk_flags) == 0:
@@ -280,13 +284,15 @@ class CardiacModel:
                 self.t = t_next
                 saver_due = _saver_due(self.state_saver, self)
                 self.t = t_keep
-            row = -1
-            if probe_open:
-                row = len(chunk_flags) if not (due_stims or extra_open or host_open) else 0
-            plain = not (due_stims or extra_open or host_open or dev_open or cmd_due or
-                         saver_due)
+            plain = not ((due_stims and not sched) or extra_open or host_open or dev_open or
+                         cmd_due or saver_due)
+            # a plain step's probe row is its place in the chunk (a host-event
+            # step runs alone, in row 0)
+            row = len(chunk_flags) if probe_open else -1
             self._t_hist.append(t)
             if plain:
+                for s in due_stims:
+                    s._host_advance(self)
                 chunk_flags.append(1 if act_open else 0)
                 chunk_rows.append(row)
                 chunk_row_owners.append(probe_open)
@@ -300,9 +306,14 @@ class CardiacModel:
             # a step with host-side work
             flush()
             chunk_start[0] = step
-            for s in due_stims:
-                s._apply_device(self, eng)
-                s.update_status(self)
+            if sched:
+                eng.apply_stims_at(step)
+                for s in due_stims:
+                    s._host_advance(self)
+            else:
+                for s in due_stims:
+                    s._apply_device(self, eng)
+                    s.update_status(self)
             # device trackers see the pre-step potential after this step's stimuli
             for tr in dev_open:
                 tr._device_track(self, eng)
@@ -343,8 +354,30 @@ class CardiacModel:
                     self.state_saver.save()
                 break
         flush()
+        if sched:
+            eng.clear_stim_schedule()
         if bar is not None:
             bar.close()
+
+    def _schedule_stims(self, eng, stims, iters) -> bool:
+        """Puts the run's stimuli on the device schedule (fw_plan_set_stims):
+        the steps each applies on, from the time fold, and its descriptor.
+        False (host-applied stimuli) when there are more than the schedule
+        holds or one has no descriptor form."""
+        from .stimulation import schedule_windows
+        eng.clear_stim_schedule()
+        if not stims or len(stims) > _native.MAX_STIMS:
+            return False
+        entries = []
+        for st, lo, hi in schedule_windows(stims, self.t, self.dt, iters):
+            e = st._sched_entry(self, eng, hi - lo + 1)
+            if e is None:
+                return False
+            entries.append((e[0], e[1], self.step + lo, self.step + hi))
+        if not entries:
+            return False
+        eng.set_stim_schedule(entries)
+        return True
 
     def _run_single(self, eng, flags, rows, owners, fused_act, probes, probe_trackers, slices,
                     pre_swap_hook=False):
@@ -353,7 +386,7 @@ class CardiacModel:
         s0 = self.step
         f = np.array(flags, dtype=np.uint8) if fused_act is not None else None
         r = np.array(rows, dtype=np.int32) if probes is not None else None
-        eng.run_steps(s0, 1, fused_act, f, probes, r, use_graph=False)
+        eng.run_steps(s0, 1, fused_act, f, probes, r, use_graph=False, stim_first=False)
         eng.swapped = not eng.swapped  # run_steps swapped; undo, caller owns it
         if probes is not None and rows[0] >= 0:
             self._collect_probes(probes, probe_trackers, slices, owners)

@@ -7,7 +7,13 @@ passed is applied at the start of a step, then ``passed = t >= stim.t +
 duration`` (stim_sequence.py:74-77, stim.py:52-56): voltage stimuli (duration
 0) fire once; current stimuli add ``dt*I`` on every step up to and including
 the first with ``t >= t0 + duration``.  Only nodes with mesh == 1 are touched.
-On the GPU each application is one small kernel over the stimulus box.
+
+On the GPU the run resolves those rules ahead of time into a device schedule
+(``schedule_windows``): each stimulus applies on a window of steps, and the
+kernel of step s applies the stimuli of step s + 1 as it writes the potential
+step s + 1 reads, so stimulated stretches need no host round trip and replay
+from captured graphs like any other.  ``_apply_device`` (one small kernel over
+the stimulus box) remains for steps the host runs one at a time.
 """
 from __future__ import annotations
 
@@ -40,6 +46,21 @@ class Stim:
     def _apply_device(self, model, eng) -> None:
         for d in self._descs(model, eng):
             eng.apply_stim(d)
+
+    # device schedule
+    def _sched_entry(self, model, eng, n_apps):
+        """(fw_stim_desc, device value list or None) for ``n_apps``
+        applications from the next one on; None if this stimulus has no
+        single-descriptor form."""
+        try:
+            descs = self._descs(model, eng)
+        except NotImplementedError:
+            return None
+        return (descs[0], None) if len(descs) == 1 else None
+
+    def _host_advance(self, model) -> None:
+        """The host bookkeeping of one application made on the device."""
+        self.update_status(model)
 
 
 class StimCurrent(Stim):
@@ -79,6 +100,33 @@ class StimSequence:
                 stim.update_status(self.model)
 
 
+def schedule_windows(stims, t, dt, iters):
+    """Steps (offsets from the current step) on which each stimulus applies
+    over the next ``iters`` steps starting at time ``t``, with the time
+    advanced as the step loop advances it (t += dt, cardiac_model.py:176):
+    a stimulus applies while ``t >= stim.t`` and it has not passed, and
+    passes once ``t >= stim.t + duration`` after an application
+    (stim_sequence.py:74-77, stim.py:52-56).  Returns [(stim, lo, hi)] for
+    the stimuli that apply, in sequence order; the applications of one
+    stimulus are consecutive steps."""
+    out = {}
+    passed = [bool(s.passed) for s in stims]
+    pending = sum(1 for p in passed if not p)
+    q = 0
+    while q < iters and pending:
+        for k, st in enumerate(stims):
+            if passed[k] or not t >= st.t:
+                continue
+            lo = out.get(k, (q, q))[0]
+            out[k] = (lo, q)
+            if t >= st.t + st.duration:
+                passed[k] = True
+                pending -= 1
+        t = t + dt
+        q += 1
+    return [(stims[k], lo, hi) for k, (lo, hi) in sorted(out.items())]
+
+
 def _box_desc(kind, box, shape, value, u_max=None, mask=None):
     d = _native.fw_stim_desc()
     d.kind = kind
@@ -88,7 +136,7 @@ def _box_desc(kind, box, shape, value, u_max=None, mask=None):
         else:
             lo, hi = 0, 1
         d.lo[a] = lo
-        d.hi[a] = hi
+        d.hi[a] = max(lo, hi)  # numpy's empty slices (hi < lo) select nothing
     d.mask = mask.data_ptr() if mask is not None else 0
     d.value = float(value)
     d.clamp = 1 if u_max is not None else 0
@@ -257,6 +305,20 @@ class StimVoltageListMatrix3D(StimVoltage, _MaskMixin):
         self.step += 1
         return [d]
 
+    def _sched_entry(self, model, eng, n_apps):
+        vals = np.asarray(self.volt_value, dtype=np.float64)[self.step:self.step + n_apps]
+        if len(vals) < n_apps:
+            # the reference's stimulate() indexes past the list on that step
+            raise IndexError(f"StimVoltageListMatrix3D: {n_apps} applications from value "
+                             f"{self.step} on, {len(np.asarray(self.volt_value))} values")
+        d = _box_desc(0, self._full_box(eng.shape), eng.shape, 0.0, None,
+                      self._mask(model, eng))
+        return d, eng.torch.from_numpy(np.ascontiguousarray(vals)).to(eng.dev)
+
+    def _host_advance(self, model) -> None:
+        self.step += 1
+        self.update_status(model)
+
 
 # ---------------------------------------------------------------- areas
 class StimCurrentArea2D(StimCurrent, _MaskMixin):
@@ -283,13 +345,18 @@ class StimCurrentArea2D(StimCurrent, _MaskMixin):
             raise ValueError("The specified area does not have healthy cells.")
         self._coords = coords[mask]
         self._mask_dev = None
+        self._mask_src = None
         return super().initialize(model)
 
-    @property
-    def matrix(self):
-        m = np.zeros(np.asarray(self._host_mesh_shape), dtype=np.uint8)
-        m[tuple(self._coords.T)] = 1
-        return m
+    def _mask(self, model, eng):
+        # the area's nodes as a device mask, built once per initialize() (the
+        # area is fixed there, stim_current_area_2d.py:67-74)
+        if self._mask_dev is None or self._mask_src != tuple(eng.shape):
+            m = np.zeros(tuple(eng.shape), dtype=np.uint8)
+            m[tuple(self._coords.T)] = 1
+            self._mask_src = tuple(eng.shape)
+            self._mask_dev = eng.torch.from_numpy(m).to(eng.dev)
+        return self._mask_dev
 
     def stimulate(self, model):
         inds = tuple(self._coords.T)
@@ -298,7 +365,6 @@ class StimCurrentArea2D(StimCurrent, _MaskMixin):
             model.u[inds] = np.where(model.u[inds] > self.u_max, self.u_max, model.u[inds])
 
     def _descs(self, model, eng):
-        self._host_mesh_shape = eng.shape
         return [_box_desc(1, self._full_box(eng.shape), eng.shape, model.dt * self.curr_value,
                           self.u_max, self._mask(model, eng))]
 

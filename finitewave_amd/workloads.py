@@ -5,8 +5,9 @@
   1 + (random <= p) with RandomState(i) (the Diffuse pattern's form), an S1
   train of 5 planar voltage beats every 400 ms on the first 5 tissue columns,
   an S2 at 1600 ms + CI, CI ~ U[150, 400] ms, horizon S2 + 500 ms;
-  activation recorded from S2 on, APD90 at the tissue centre, conduction
-  velocity along the centre row.
+  activation recorded from S2 on, APD90 at the tissue centre on the trace
+  from S2 on (no upstroke there = loss of capture), conduction velocity as
+  the reference's front velocity and along the centre row.
 * ``serving_stream``: Poisson arrivals (rate 200/s), TP06 or Aliev-Panfilov
   with probability 1/2, side ~ U{128..1024}, fibrosis p ~ U[0, 0.4], uniform
   fibre angle ~ U[0, 180) (9-point stencil), planar or S1-S2 cross-field
@@ -62,7 +63,7 @@ def restitution_ensemble(n: int = 512, seed: int = 1234, sides=(64, 128, 256, 51
         mid = (s + 2) // 2
         sims.append(BatchSim(tissue=tissue, t_max=s2 + tail, stims=stims,
                              probe=_tissue_cell(tissue.mesh, mid, mid), act_threshold=0.5,
-                             act_start=s2, apd_threshold=0.5, cv_row=mid,
+                             act_start=s2, apd_threshold=0.5, apd_start=s2, cv_row=mid,
                              tag=dict(side=s, p=float(p[i]), ci=float(ci[i]))))
     return MitchellSchaeffer2D, sims
 

@@ -39,6 +39,11 @@
  *                          kernel per step, weights rebuilt from the mesh.
  *   fw_stim_apply       <- Stim*.stimulate (cpuwave2D/stimulation/*.py,
  *                          cpuwave3D/stimulation/*.py)
+ *   fw_plan_set_stims / fw_stim_apply_step
+ *                       <- StimSequence.stimulate_next + Stim.update_status
+ *                          (core/stimulation/stim_sequence.py:65-77,
+ *                          core/stimulation/stim.py:52-56) over a run, as a
+ *                          device schedule fused into the step kernel
  *   fw_act_time_update  <- ActivationTime2DTracker._track
  *                          (cpuwave2D/tracker/activation_time_2d_tracker.py:61-63)
  *   fw_lat_update       <- LocalActivationTime2DTracker._track
@@ -64,7 +69,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 1
+#define FW_ABI_VERSION 2
 
 /* status codes */
 #define FW_OK 0
@@ -153,6 +158,11 @@ typedef struct fw_track_desc {
   const int32_t* probe_var;
   double* probe_buf;
   const int32_t* probe_rows;
+  /* the plan's scheduled stimuli (fw_plan_set_stims) of every step s + 1 with
+   * s + 1 < stim_end are applied by the kernel of step s as it writes u_new;
+   * stim_end <= step0 + 1 (e.g. 0) applies none.  The stimuli of the first
+   * step of a run are the caller's: fw_stim_apply_step. */
+  int64_t stim_end;
 } fw_track_desc;
 
 typedef struct fw_stim_desc {
@@ -221,8 +231,44 @@ int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state,
                  int64_t step0, int32_t n_steps, const fw_track_desc* track,
                  void* stream);
 
+/* Captures the graphs fw_run_graph would replay for the same arguments
+ * (buffers, tracker, the plan's stimulus schedule, n_steps) and launches
+ * nothing: capture and instantiation, which take milliseconds, then happen
+ * outside a caller's timed or latency-bound region.  No counterpart in the
+ * reference, which has no graphs. */
+int fw_run_graph_prepare(fw_plan* plan, void* u, void* u_new, void* const* state,
+                         int32_t n_steps, const fw_track_desc* track);
+
 int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim,
                   void* stream);
+
+/* ---- stimulus schedule ------------------------------------------------------
+ * StimSequence.stimulate_next (core/stimulation/stim_sequence.py:65-77) with
+ * Stim.update_status (core/stimulation/stim.py:52-56) resolved ahead of time:
+ * a stimulus with t >= stim.t that has not passed is applied at the start of
+ * a step, then passes once t >= stim.t + duration, so with the FP64 time
+ * fold t_s it is applied on the steps [step_lo, step_hi] (global step
+ * numbers).  Applied to the potential before the step, in schedule order, to
+ * nodes with FW_CODE_CONDUCT inside the box and the mask.  values: device
+ * double [step_hi - step_lo + 1] or NULL; for a voltage stimulus, the value
+ * of each step (StimVoltageListMatrix3D.stimulate,
+ * cpuwave3D/stimulation/stim_voltage_list_matrix_3d.py:60-72). */
+#define FW_MAX_STIMS 8
+typedef struct fw_stim_sched {
+  fw_stim_desc stim;
+  int64_t step_lo, step_hi;
+  const double* values;
+} fw_stim_sched;
+
+/* Replaces the plan's schedule with n (<= FW_MAX_STIMS) entries, copied;
+ * masks and value lists stay the caller's device memory.  Graphs captured
+ * with another schedule are dropped. */
+int fw_plan_set_stims(fw_plan* plan, const fw_stim_sched* stims, int32_t n);
+
+/* Applies the scheduled stimuli due at `step` to u (one launch each): the
+ * first step of an fw_run / fw_run_graph call, whose kernels apply those of
+ * the later steps. */
+int fw_stim_apply_step(fw_plan* plan, void* u, int64_t step, void* stream);
 
 /* One activation-time pass over all N nodes (act = step where u > thr first) */
 int fw_act_time_update(fw_plan* plan, int32_t* act, const void* u, double thr,

@@ -13,6 +13,7 @@ import numpy as np
 import pytest
 
 import finitewave_amd as fw
+from finitewave_amd.analysis import apd90
 from finitewave_amd.batch import BatchSim, TissueBatch
 from finitewave_amd.serving import Request, Server
 from finitewave_amd.workloads import fibrotic_tissue, restitution_ensemble
@@ -184,6 +185,34 @@ def test_restitution_ensemble_sample(gpu):
     assert apds[-1] > apds[0]
     ref = _single(cls, "fp32", sims[0])
     _check_same(sims[0].result, ref)
+
+
+@pytest.mark.gpu
+def test_restitution_loss_of_capture(gpu):
+    """An S2 inside the refractory period of the last S1 beat produces no
+    upstroke at the probe: APD90 is None and the tissue is reported as not
+    captured (RestitutionCurve.ipynb cell 6), while a late S2 is captured and
+    measured on its own beat; the front velocity is found for both S1-S2
+    pairs that activate the tissue."""
+    # a 500 ms cycle: both S1 beats capture (the model's APD is shorter)
+    cls, sims = restitution_ensemble(n=2, sides=(32,), beats=2, bcl=500.0, tail=300.0,
+                                     p_range=(0.0, 0.0), ci_range=(30.0, 30.0))
+    _, late = restitution_ensemble(n=1, sides=(32,), beats=2, bcl=500.0, tail=400.0,
+                                   p_range=(0.0, 0.0), ci_range=(450.0, 450.0))
+    b = TissueBatch(cls, precision="fp32", max_tissues=4)
+    for s in sims + late:
+        b.admit(s)
+    b.run_all()
+    b.close()
+    for s in sims:
+        # the last S1 beat was captured at the probe (the whole trace holds
+        # an upstroke), the S2 was not
+        assert apd90(s.result["trace"], 0.01, 0.5) is not None
+        assert s.result["captured"] is False and s.result["apd90"] is None
+    r = late[0].result
+    assert r["captured"] is True and r["apd90"] is not None and r["apd90"] > 50
+    # the S1 beats are in the trace too, but APD90 is the S2 beat's
+    assert r["cv_front"] is not None and r["cv_front"] > 0
 
 
 @pytest.mark.gpu

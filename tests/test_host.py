@@ -25,7 +25,7 @@ def test_library_loads_and_exports_every_declared_symbol():
     L = _native.lib()
     for name in declared:
         assert hasattr(L, name), name
-    assert L.fw_abi_version() == 1
+    assert L.fw_abi_version() == _native.ABI_VERSION
     assert _native.model_info(_native.MODEL_IDS["tp06"]) == (19, 49)
     assert _native.model_info(_native.MODEL_IDS["courtemanche"]) == (21, 39)
     # TP06's cai/casr/cass/nai/Ki and sodium gates h, j stay FP64 on the FP32 path
@@ -166,7 +166,8 @@ def test_model_classes_keep_reference_attributes():
 
 
 # ---------------------------------------------------------------- batches (host side)
-from finitewave_amd.analysis import apd90, conduction_velocity  # noqa: E402
+from finitewave_amd.analysis import (apd90, captured, conduction_velocity,  # noqa: E402
+                                     front_velocity)
 from finitewave_amd.batch import (BatchSim, _uniform_fields, run_length,  # noqa: E402
                                   stim_schedule, time_fold)
 
@@ -275,6 +276,48 @@ def test_apd90_closed_form_on_a_synthetic_trace():
     assert apd90(jump, dt, 0.5) == 0.0
 
 
+def test_apd90_of_the_s2_segment_and_loss_of_capture():
+    """RestitutionCurve.ipynb cell 6 measures the S2 run's trace only: an S2
+    that produces no upstroke is a loss of capture (None), not the APD90 of
+    the S1 beat before it."""
+    dt = 0.1
+    t = np.arange(0, 900, dt)
+    beat = np.clip(np.minimum((t - 100) / 2, 1 - (t - 102) / 300), 0, 1) * (t >= 100)
+    s2 = int(round(500 / dt))
+    # S1 beat at 100 ms, then a sub-threshold bump at 500 ms (S2 not captured)
+    bump = 0.3 * np.exp(-np.maximum(t - 500, 0) / 5) * (t >= 500)
+    u = beat + bump
+    assert apd90(u, dt, 0.5) is not None          # the whole trace holds the S1 beat
+    assert apd90(u, dt, 0.5, start=s2) is None    # the S2 segment does not
+    assert not captured(u, 0.5, start=s2) and captured(u, 0.5)
+    # a captured S2: its own beat is measured
+    t2 = t - 400
+    beat2 = np.clip(np.minimum((t2 - 100) / 2, 1 - (t2 - 102) / 200), 0, 1) * (t2 >= 100)
+    v = np.where(t < 450, beat, beat2)
+    assert captured(v, 0.5, start=s2)
+    assert abs(apd90(v, dt, 0.5, start=s2) - (102 + 0.9 * 200 - 100.9)) <= 2 * dt
+
+
+def test_front_velocity_closed_forms():
+    """Velocity2DCalculation.front_velocity (velocity_2d_calculation.py:15-44)
+    on maps whose answer is known: a planar front, a point source (distance
+    from the source to the far corners), nodes that never activated."""
+    dr = 0.25
+    act = np.tile(np.arange(40) * 0.5, (6, 1)) + 3.0  # 0.5 ms per node, from t = 3
+    v = front_velocity(act, dr)
+    assert v.shape == (6,) and np.allclose(v, dr / 0.5, rtol=0, atol=1e-12)
+    act[:, 0] = -1  # never activated: ignored for the earliest time
+    assert np.allclose(front_velocity(act, dr), dr / 0.5)
+    i, j = np.meshgrid(np.arange(21), np.arange(21), indexing="ij")
+    ring = np.hypot(i - 10, j - 10)
+    pt = np.round(ring, 6) * 2.0        # a point source at (10, 10), 2 ms per node
+    v = front_velocity(pt, dr)
+    assert np.allclose(v, dr / 2.0)     # four corners, each sqrt(200) nodes away
+    assert len(v) == 4
+    assert front_velocity(-np.ones((4, 4)), dr) is None
+    assert front_velocity(np.zeros((4, 4)), dr) is None
+
+
 def test_conduction_velocity_closed_form():
     dr = 0.25
     act = np.tile(np.arange(41) * 0.5, (3, 1))      # 0.5 ms per node
@@ -372,3 +415,52 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode == 2
     assert "one GPU" in (r.stdout + r.stderr)
     assert r.stdout.strip() == "" or not r.stdout.lstrip().startswith("{")
+
+
+def test_schedule_windows_follow_the_stimulus_rules():
+    """schedule_windows resolves stimulate_next + update_status
+    (stim_sequence.py:74-77, stim.py:52-56) ahead of time: the same steps as
+    applying the rules step by step with the FP64 time fold, on random
+    sequences; and the fenceposts of the time fold: a 0.5 ms current from
+    t = 0 at dt 0.01 applies 51 times, an S2 at 310 ms first applies on step
+    31,001 (t after 31,000 steps is 309.99999999986306)."""
+    from finitewave_amd.stimulation import schedule_windows
+
+    class _M:
+        t = 0.0
+
+    rng = np.random.RandomState(0)
+    dt = 0.01
+    for trial in range(60):
+        stims = []
+        for _ in range(rng.randint(1, 5)):
+            t0 = round(float(rng.uniform(0, 2)), int(rng.randint(0, 4)))
+            dur = 0.0 if rng.rand() < 0.3 else round(float(rng.uniform(0, 1)), int(rng.randint(0, 3)))
+            stims.append(fw.StimCurrentCoord2D(t0, 1.0, dur, 0, 1, 0, 1))
+        t_start = 0.0 if trial % 3 else float(rng.uniform(0, 1))
+        iters = int(rng.randint(1, 300))
+        got = [(stims.index(s), lo, hi) for s, lo, hi in schedule_windows(stims, t_start, dt, iters)]
+        # step by step, the reference's rules on copies
+        m = _M()
+        t = t_start
+        apps = {}
+        for q in range(iters):
+            m.t = t
+            for k, s in enumerate(stims):
+                if t >= s.t and not s.passed:
+                    apps.setdefault(k, []).append(q)
+                    s.update_status(m)
+            t = t + dt
+        want = [(k, a[0], a[-1]) for k, a in sorted(apps.items())]
+        for k, a in apps.items():
+            assert a == list(range(a[0], a[-1] + 1))
+        assert got == want, trial
+    s = fw.StimCurrentCoord2D(0, 1.0, 0.5, 0, 1, 0, 1)
+    assert [(lo, hi) for _, lo, hi in schedule_windows([s], 0.0, dt, 1000)] == [(0, 50)]
+    s2 = fw.StimCurrentCoord2D(310, 100, 1.0, 0, 1, 0, 1)
+    [(_, lo, hi)] = schedule_windows([s2], 0.0, dt, 32000)
+    assert lo == 31001
+    t = 0.0
+    for _ in range(31000):
+        t += dt
+    assert t == 309.99999999986306

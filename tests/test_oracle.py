@@ -187,6 +187,61 @@ def test_aniso_transpose_and_mirror_symmetry():
     assert np.allclose(wa[::-1, :, ::-1, :][inner], wc[inner], atol=1e-15)
 
 
+# asymmetric_stencil_3d.py:26-44: offset (di, dj, dk) of each of the 19 weights
+_W19 = [(-1, -1, 0), (-1, 0, 0), (-1, 1, 0), (0, -1, 0), (0, 0, 0), (0, 1, 0), (1, -1, 0),
+        (1, 0, 0), (1, 1, 0), (0, -1, -1), (0, -1, 1), (0, 0, -1), (0, 0, 1), (0, 1, -1),
+        (0, 1, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1), (1, 0, 1)]
+
+
+def _block19(w):
+    """(*shape, 19) weights as (*shape, 3, 3, 3) neighbourhood blocks."""
+    out = np.zeros(w.shape[:3] + (3, 3, 3))
+    for q, (a, b, c) in enumerate(_W19):
+        out[..., a + 1, b + 1, c + 1] = w[..., q]
+    return out
+
+
+@pytest.mark.parametrize("perm", [(1, 0, 2), (2, 1, 0), (0, 2, 1)])
+def test_aniso3d_axis_swap_symmetry(perm):
+    """Exchanging two grid axes and the same two fibre components exchanges
+    the same two axes of every node's 19-point block
+    (asymmetric_stencil_3d.py:163-458), cross terms involving z included."""
+    shape = (9, 10, 11)
+    mesh = boxed_mesh(shape, 0.15, seed=4)
+    f = random_fibers(shape, seed=8)
+    a = OracleSim("aliev_panfilov", mesh, 0.01, 0.25, fibers=f, stencil="aniso")
+    a.compute_weights()
+    b = OracleSim("aliev_panfilov", np.ascontiguousarray(mesh.transpose(perm)), 0.01, 0.25,
+                  fibers=np.ascontiguousarray(f.transpose(*perm, 3)[..., list(perm)]),
+                  stencil="aniso")
+    b.compute_weights()
+    wa = _block19(a.weights.reshape(*shape, 19)).transpose(*perm, *(3 + np.array(perm)))
+    wb = _block19(b.weights.reshape(*b.mesh.shape, 19))
+    inner = (slice(1, -1),) * 3
+    assert np.abs(wa[inner]).max() > 0.01
+    assert np.allclose(wa[inner], wb[inner], atol=1e-15, rtol=0)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_aniso3d_mirror_symmetry(axis):
+    """Mirroring the grid along one axis and negating that fibre component
+    mirrors every node's 19-point block along that axis."""
+    shape = (9, 10, 11)
+    mesh = boxed_mesh(shape, 0.15, seed=6)
+    f = random_fibers(shape, seed=9)
+    a = OracleSim("aliev_panfilov", mesh, 0.01, 0.25, fibers=f, stencil="aniso")
+    a.compute_weights()
+    fm = np.flip(f, axis).copy()
+    fm[..., axis] *= -1
+    b = OracleSim("aliev_panfilov", np.ascontiguousarray(np.flip(mesh, axis)), 0.01, 0.25,
+                  fibers=fm, stencil="aniso")
+    b.compute_weights()
+    wa = np.flip(np.flip(_block19(a.weights.reshape(*shape, 19)), axis), 3 + axis)
+    wb = _block19(b.weights.reshape(*shape, 19))
+    inner = (slice(1, -1),) * 3
+    assert np.allclose(wa[inner], wb[inner], atol=1e-15, rtol=0)
+
+
 def test_aniso3d_reduces_to_2d_in_a_plane():
     # fibres in the xy-plane and a slab one tissue node thick in z: the
     # 19-point weights restricted to the plane equal the 9-point weights
