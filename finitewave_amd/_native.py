@@ -22,6 +22,7 @@ PREC_FP64, PREC_FP32 = 0, 1
 FIELD_UNIFORM, FIELD_AXIS, FIELD_FULL = 0, 1, 2
 CODE_CONDUCT, CODE_FROZEN, CODE_PROBE = 1, 2, 4
 MAX_STATE, MAX_PARAMS, MAX_FIELDS, MAX_STIMS = 24, 64, 9, 8
+NOALLOC_SCRATCH_BYTES = 4 << 20
 ABI_VERSION = 2
 
 # every symbol include/finitewave_amd.h declares
@@ -29,7 +30,7 @@ EXPORTS = (
     "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64", "fw_u_is_f64",
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
-    "fw_run_graph_prepare",
+    "fw_run_graph_prepare", "fw_run_noalloc",
     "fw_stim_apply", "fw_plan_set_stims", "fw_stim_apply_step", "fw_act_time_update",
     "fw_lat_update", "fw_spiral_tips", "fw_ecg",
     "fw_fibrosis",
@@ -109,6 +110,8 @@ def lib():
     L.fw_run.argtypes = [vp, vp, vp, POINTER(vp), i64, i32, POINTER(fw_track_desc), vp]
     L.fw_run_graph.argtypes = [vp, vp, vp, POINTER(vp), i64, i32, POINTER(fw_track_desc), vp]
     L.fw_run_graph_prepare.argtypes = [vp, vp, vp, POINTER(vp), i32, POINTER(fw_track_desc)]
+    L.fw_run_noalloc.argtypes = [POINTER(fw_plan_desc), vp, vp, POINTER(vp), i64, i32,
+                                 POINTER(fw_track_desc), vp]
     L.fw_stim_apply.argtypes = [vp, vp, POINTER(fw_stim_desc), vp]
     L.fw_plan_set_stims.argtypes = [vp, POINTER(fw_stim_sched), i32]
     L.fw_stim_apply_step.argtypes = [vp, vp, i64, vp]

@@ -4,6 +4,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "fw_batch.hpp"
 #include "fw_internal.hpp"
 
@@ -544,6 +546,117 @@ int fw_run(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step
   int rc = check_run_args(plan, u, u_new, state, n_steps, track);
   if (rc) return rc;
   return run_plain(plan, u, u_new, state, step0, n_steps, track, stream);
+}
+
+// ---------------------------------------------------------------------------
+// Stepping without allocation (fw_run_noalloc)
+// ---------------------------------------------------------------------------
+
+// The staging memory of fw_run_noalloc: the float copies and along-k tables
+// of a plan's half-step fields.  One copy per GPU, held by the loaded code
+// object until the process ends.  In host code the name is a stand-in, not a
+// device address: host code uses only the address hipGetSymbolAddress gives
+// for the current device (scratch_for_current_device).
+static __device__ __align__(256) unsigned char fw_scratch[FW_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices];
+
+// The current device's address of fw_scratch, looked up on the first call on
+// that device and kept
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    set_error("fw_run_noalloc: no usable current device (%s, %d)", hipGetErrorString(e), dev);
+    return FW_E_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(fw_scratch));
+    if (e != hipSuccess || !p) {
+      set_error("fw_run_noalloc: hipGetSymbolAddress failed on device %d: %s", dev,
+                hipGetErrorString(e));
+      return FW_E_HIP;
+    }
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return FW_OK;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int fw_run_noalloc(const fw_plan_desc* desc, void* u, void* u_new, void* const* state,
+                   int64_t step0, int32_t n_steps, const fw_track_desc* track, void* stream) {
+  if (!desc) {
+    set_error("fw_run_noalloc: null plan description");
+    return FW_E_INVALID;
+  }
+  fw_plan p{};
+  int rc = plan_host_init(desc, &p, "fw_run_noalloc");
+  if (rc) return rc;
+  rc = check_run_args(&p, u, u_new, state, n_steps, track);
+  if (rc) return rc;
+  const fw_plan_desc& d = p.desc;
+  const int need = needed_fields(d);
+  // offsets of the tables in the scratch array, then its size
+  size_t off_f[FW_MAX_FIELDS] = {}, off_kd[FW_MAX_FIELDS] = {}, off_kf[FW_MAX_FIELDS] = {};
+  int64_t len_f[FW_MAX_FIELDS] = {};
+  size_t used = 0;
+  const bool ftabs = wants_float_tables(d);
+  for (int q = 0; q < need; ++q) {
+    const fw_field& f = d.fields[q];
+    if (!ftabs || f.mode == FW_FIELD_UNIFORM) continue;
+    len_f[q] = f.mode == FW_FIELD_AXIS ? d.shape[f.axis] : p.N;
+    off_f[q] = used;
+    used = align256(used + (size_t)len_f[q] * sizeof(float));
+  }
+  const int64_t nk = d.shape[2];
+  if (p.field_layout == kFieldsK) {
+    for (int q = 0; q < need; ++q) {
+      off_kd[q] = used;
+      used = align256(used + (size_t)nk * sizeof(double));
+      off_kf[q] = used;
+      used = align256(used + (size_t)nk * sizeof(float));
+    }
+  }
+  if (used > FW_NOALLOC_SCRATCH_BYTES) {
+    set_error("fw_run_noalloc: the plan's field tables need %zu bytes, more than the %u bytes "
+              "of static scratch; use fw_plan_create",
+              used, (unsigned)FW_NOALLOC_SCRATCH_BYTES);
+    return FW_E_INVALID;
+  }
+  hipStream_t s = as_stream(stream);
+  if (used > 0) {
+    void* base = nullptr;
+    rc = scratch_for_current_device(&base);
+    if (rc) return rc;
+    unsigned char* b = static_cast<unsigned char*>(base);
+    for (int q = 0; q < need; ++q) {
+      if (len_f[q] > 0) {
+        p.ftab[q] = reinterpret_cast<float*>(b + off_f[q]);
+        hipLaunchKernelGGL(k_to_float, dim3((unsigned)((len_f[q] + 255) / 256)), dim3(256), 0,
+                           s, d.fields[q].data, p.ftab[q], len_f[q]);
+      }
+      if (p.field_layout == kFieldsK) {
+        const fw_field& f = d.fields[q];
+        p.ktab[q] = reinterpret_cast<double*>(b + off_kd[q]);
+        p.fktab[q] = reinterpret_cast<float*>(b + off_kf[q]);
+        hipLaunchKernelGGL(k_fill_ktab, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s,
+                           f.mode == FW_FIELD_AXIS ? f.data : nullptr, f.value, p.ktab[q],
+                           p.fktab[q], nk);
+      }
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+      set_error("fw_run_noalloc: table launch failed: %s", hipGetErrorString(e));
+      return FW_E_HIP;
+    }
+  }
+  return run_plain(&p, u, u_new, state, step0, n_steps, track, stream);
 }
 
 // Pairs of steps per launch of the long graph: a graph launch costs several

@@ -239,6 +239,24 @@ int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state,
 int fw_run_graph_prepare(fw_plan* plan, void* u, void* u_new, void* const* state,
                          int32_t n_steps, const fw_track_desc* track);
 
+/* Same as fw_plan_create + fw_run + fw_plan_destroy with no stimulus
+ * schedule, calling no hipMalloc, hipFree or other allocating routine: for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation.  No
+ * counterpart in the reference.  The float copies and along-k tables of the
+ * description's half-step fields (what fw_plan_create allocates) are staged
+ * in a static device array of FW_NOALLOC_SCRATCH_BYTES on each GPU, kept
+ * until the process ends; a description whose tables need more is refused
+ * with FW_E_INVALID before any device work.  Every call on a GPU shares that
+ * GPU's array and rewrites it before its steps read it: calls that overlap
+ * on one GPU (different streams or threads) race and must be serialised by
+ * the caller.  A FW_STENCIL_WEIGHTS description uses 9 (2D) / 19 (3D)
+ * weights per node. */
+#define FW_NOALLOC_SCRATCH_BYTES (4u << 20)
+int fw_run_noalloc(const fw_plan_desc* desc, void* u, void* u_new, void* const* state,
+                   int64_t step0, int32_t n_steps, const fw_track_desc* track,
+                   void* stream);
+
 int fw_stim_apply(fw_plan* plan, void* u, const fw_stim_desc* stim,
                   void* stream);
 
