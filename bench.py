@@ -116,7 +116,7 @@ def gpu_run(precision: str, steps: int, warmup: int):
     s1 = [(e.step_lo, e.step_hi) for e in eng.stim_sched]
     if warmup:
         eng.run_steps(0, warmup, tracker, np.ones(warmup, np.uint8), use_graph=True)
-    eng.prepare_graphs(steps - (1 if eng.swapped else 0), tracker)
+    eng.prepare_graphs(steps, tracker)
     torch.cuda.synchronize()
     flags = np.ones(steps, np.uint8)
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)

@@ -347,8 +347,6 @@ static int plan_host_init(const fw_plan_desc* desc, fw_plan* p, const char* who)
   p->per_xcd = (p->nblocks + kXcds - 1) / kXcds;
   p->grid = p->per_xcd * kXcds;
   p->K = d.ndim == 2 ? 9 : 19;
-  p->graph = nullptr;
-  p->graph_steps = 0;
   // the field layout: every field uniform (kFieldsUniform), or in 3D every
   // field uniform or tabulated along the last axis (kFieldsK, layered
   // fibres); such plans get their step kernel without the per-field mode
@@ -468,10 +466,22 @@ int fw_plan_set_weights_k(fw_plan* plan, int32_t K) {
   return FW_OK;
 }
 
+// Drops every captured step graph of a plan.
+static void drop_graphs(fw_plan* plan) {
+  for (auto& gs : plan->graphs) {
+    for (auto& g : gs.g) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+    gs.u = gs.un = nullptr;
+    gs.last_use = 0;
+  }
+  plan->graph_keyed = 0;
+}
+
 int fw_plan_destroy(fw_plan* plan) {
   if (!plan) return FW_OK;
-  if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
-  if (plan->graph_long) (void)hipGraphExecDestroy(plan->graph_long);
+  drop_graphs(plan);
   if (plan->step_words) (void)hipFree(plan->step_words);
   for (int q = 0; q < FW_MAX_FIELDS; ++q) {
     if (plan->ftab[q]) (void)hipFree(plan->ftab[q]);
@@ -659,12 +669,16 @@ int fw_run_noalloc(const fw_plan_desc* desc, void* u, void* u_new, void* const* 
   return run_plain(&p, u, u_new, state, step0, n_steps, track, stream);
 }
 
-// Pairs of steps per launch of the long graph: a graph launch costs several
-// microseconds on the host, as much as a whole step of a small grid, so long
-// runs replay 64 steps per launch.
-constexpr int32_t kGraphLongPairs = 32;
+// Graphs of 1, 2, 4, ..., 2^(kGraphBits-1) pairs of steps: a run of n steps
+// replays n / 2 pairs as whole 32-pair graphs and then the binary digits of
+// the remainder, so a replay costs few graph launches (each costs several
+// microseconds on the host and leaves a gap of ~9 us between launches on the
+// device, as much as a fifth of a small grid's step).
+constexpr int kGraphBits = 6;
+constexpr int32_t kGraphLongPairs = 1 << (kGraphBits - 1);
 
-// Captures `pairs` pairs of steps (u -> u_new -> u) into an executable graph.
+// Captures `pairs` pairs of steps (u -> u_new -> u) into an executable graph
+// and uploads it, so its first launch costs no more than the next ones.
 // Each kernel reads its step from one device word and writes step + 1 to the
 // other, so a replay continues from whatever step the words hold.
 static int capture_pairs(fw_plan* plan, void* u, void* u_new, const StatePtrs& st,
@@ -699,8 +713,11 @@ static int capture_pairs(fw_plan* plan, void* u, void* u_new, const StatePtrs& s
   hipGraphExec_t ex = nullptr;
   if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
   if (g) (void)hipGraphDestroy(g);
+  if (e == hipSuccess && !rc && ex) e = hipGraphUpload(ex, cap);
+  if (e == hipSuccess && !rc && ex) e = hipStreamSynchronize(cap);
   (void)hipStreamDestroy(cap);
   if (e != hipSuccess || rc || !ex) {
+    if (ex) (void)hipGraphExecDestroy(ex);
     set_error("fw_run_graph: capture failed (%s)", hipGetErrorString(e));
     return rc ? rc : FW_E_HIP;
   }
@@ -708,10 +725,12 @@ static int capture_pairs(fw_plan* plan, void* u, void* u_new, const StatePtrs& s
   return FW_OK;
 }
 
-// The graphs fw_run_graph replays for these buffers, tracker and schedule:
-// captured now unless the cached ones match.  *act_on_out: the gate.
+// The graph set fw_run_graph replays for these buffers, tracker and schedule
+// (the plan's schedule is in every captured kernel's arguments; changing it
+// drops the graphs), with the graphs an n_steps run needs captured.
 static int ensure_graphs(fw_plan* plan, void* u, void* u_new, void* const* state,
-                         int32_t n_steps, const fw_track_desc* track, int32_t* act_on_out) {
+                         int32_t n_steps, const fw_track_desc* track, int32_t* act_on_out,
+                         fw_plan::GraphSet** set_out) {
   int rc = check_run_args(plan, u, u_new, state, n_steps, track);
   if (rc) return rc;
   if (track && track->n_probe > 0) {
@@ -733,66 +752,79 @@ static int ensure_graphs(fw_plan* plan, void* u, void* u_new, void* const* state
   *act_on_out = act_on;
   const double act_thr = track ? track->act_thr : 0.0;
   const int ns = model_n_state(plan->desc.model);
-  const int32_t pairs = n_steps / 2;
-  // cached graphs still valid for these buffers?
-  bool reuse = plan->graph && plan->graph_u == u && plan->graph_un == u_new &&
-               plan->graph_act == act && plan->graph_act_on == act_on &&
-               (!act || plan->graph_act_thr == act_thr);
-  for (int k = 0; reuse && k < ns; ++k) reuse = plan->graph_state[k] == state[k];
-  const StatePtrs st = make_state(state, ns);
-  if (!reuse && pairs > 0) {
-    if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
-    if (plan->graph_long) (void)hipGraphExecDestroy(plan->graph_long);
-    plan->graph = nullptr;
-    plan->graph_long = nullptr;
-    rc = capture_pairs(plan, u, u_new, st, act, act_on, act_thr, 1, &plan->graph);
-    if (rc) return rc;
-    plan->graph_u = u;
-    plan->graph_un = u_new;
+  // graphs captured for another tracker or state are of no use
+  bool same = plan->graph_keyed && plan->graph_act == act && plan->graph_act_on == act_on &&
+              (!act || plan->graph_act_thr == act_thr);
+  for (int k = 0; same && k < ns; ++k) same = plan->graph_state[k] == state[k];
+  if (!same) {
+    drop_graphs(plan);
     plan->graph_act = act;
     plan->graph_act_on = act_on;
     plan->graph_act_thr = act_thr;
     for (int k = 0; k < FW_MAX_STATE; ++k) plan->graph_state[k] = k < ns ? state[k] : nullptr;
+    plan->graph_keyed = 1;
   }
-  if (pairs >= kGraphLongPairs && !plan->graph_long) {
-    rc = capture_pairs(plan, u, u_new, st, act, act_on, act_thr, kGraphLongPairs,
-                       &plan->graph_long);
-    if (rc) return rc;
+  // the orientation's set, or the least recently used one
+  fw_plan::GraphSet* gs = nullptr;
+  for (auto& cand : plan->graphs)
+    if (cand.u == u && cand.un == u_new) gs = &cand;
+  if (!gs) {
+    gs = plan->graphs[0].last_use <= plan->graphs[1].last_use ? &plan->graphs[0]
+                                                               : &plan->graphs[1];
+    for (auto& g : gs->g) {
+      if (g) (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+    gs->u = u;
+    gs->un = u_new;
   }
+  gs->last_use = ++plan->graph_clock;
+  const int32_t pairs = n_steps / 2;
+  const StatePtrs st = make_state(state, ns);
+  for (int bit = 0; bit < kGraphBits; ++bit) {
+    const int32_t size = 1 << bit;
+    const bool needed = bit == kGraphBits - 1 ? pairs >= size : ((pairs % kGraphLongPairs) & size);
+    if (needed && !gs->g[bit]) {
+      rc = capture_pairs(plan, u, u_new, st, act, act_on, act_thr, size, &gs->g[bit]);
+      if (rc) return rc;
+    }
+  }
+  *set_out = gs;
   return FW_OK;
 }
 
 int fw_run_graph_prepare(fw_plan* plan, void* u, void* u_new, void* const* state,
                          int32_t n_steps, const fw_track_desc* track) {
   int32_t act_on = 0;
-  return ensure_graphs(plan, u, u_new, state, n_steps, track, &act_on);
+  fw_plan::GraphSet* gs = nullptr;
+  return ensure_graphs(plan, u, u_new, state, n_steps, track, &act_on, &gs);
 }
 
 int fw_run_graph(fw_plan* plan, void* u, void* u_new, void* const* state, int64_t step0,
                  int32_t n_steps, const fw_track_desc* track, void* stream) {
   int32_t act_on = 0;
-  int rc = ensure_graphs(plan, u, u_new, state, n_steps, track, &act_on);
+  fw_plan::GraphSet* gs = nullptr;
+  int rc = ensure_graphs(plan, u, u_new, state, n_steps, track, &act_on, &gs);
   if (rc) return rc;
   const int32_t pairs = n_steps / 2;
   hipStream_t s = as_stream(stream);
   if (pairs > 0) {
     hipLaunchKernelGGL(k_set_run_words, dim3(1), dim3(1), 0, s, plan->step_words,
                        (int32_t)step0, track ? clamp_step(track->stim_end) : 0);
-    int32_t q = 0;
-    for (; q + kGraphLongPairs <= pairs; q += kGraphLongPairs) {
-      hipError_t e = hipGraphLaunch(plan->graph_long, s);
+    auto launch = [&](hipGraphExec_t g) -> int {
+      const hipError_t e = hipGraphLaunch(g, s);
       if (e != hipSuccess) {
         set_error("fw_run_graph: launch failed: %s", hipGetErrorString(e));
         return FW_E_HIP;
       }
-    }
-    for (; q < pairs; ++q) {
-      hipError_t e = hipGraphLaunch(plan->graph, s);
-      if (e != hipSuccess) {
-        set_error("fw_run_graph: launch failed: %s", hipGetErrorString(e));
-        return FW_E_HIP;
-      }
-    }
+      return FW_OK;
+    };
+    for (int32_t q = 0; q + kGraphLongPairs <= pairs; q += kGraphLongPairs)
+      if ((rc = launch(gs->g[kGraphBits - 1]))) return rc;
+    const int32_t rest = pairs % kGraphLongPairs;
+    for (int bit = kGraphBits - 2; bit >= 0; --bit)
+      if ((rest >> bit) & 1)
+        if ((rc = launch(gs->g[bit]))) return rc;
   }
   if (n_steps % 2) {
     // the last odd step runs as a plain launch on the original orientation
@@ -851,10 +883,7 @@ int fw_plan_set_stims(fw_plan* plan, const fw_stim_sched* stims, int32_t n) {
                     (n == 0 || memcmp(plan->stims, stims, n * sizeof(fw_stim_sched)) == 0);
   if (!same) {
     // captured graphs hold the old schedule in their kernel arguments
-    if (plan->graph) (void)hipGraphExecDestroy(plan->graph);
-    if (plan->graph_long) (void)hipGraphExecDestroy(plan->graph_long);
-    plan->graph = nullptr;
-    plan->graph_long = nullptr;
+    drop_graphs(plan);
     plan->n_stims = n;
     if (n > 0) memcpy(plan->stims, stims, n * sizeof(fw_stim_sched));
   }

@@ -15,15 +15,21 @@ struct fw_plan {
   uint32_t grid;     // launched blocks = per_xcd * 8
   int32_t K;         // weights per node for FW_STENCIL_WEIGHTS
   int32_t* step_words;  // device int32[4]: graph step words [0..1], stim_end [2]
-  hipGraphExec_t graph;       // two steps (u -> u_new -> u)
-  hipGraphExec_t graph_long;  // kGraphLongPairs such pairs, for long runs
-  int32_t graph_steps;
-  const void* graph_u;
-  const void* graph_un;
+  // captured step graphs for two buffer orientations (u, u_new) and (u_new,
+  // u): per orientation, graphs of 2^b pairs of steps (b = 0..kGraphBits-1),
+  // captured when first needed; all keyed on the same tracker and state
+  struct GraphSet {
+    const void* u;
+    const void* un;
+    hipGraphExec_t g[6];
+    uint32_t last_use;
+  } graphs[2];
+  uint32_t graph_clock;
   void* graph_state[FW_MAX_STATE];
   int32_t graph_act_on;
   int32_t* graph_act;
   double graph_act_thr;
+  int32_t graph_keyed;  // graph_state/act/act_on/act_thr hold a key
   // the stimulus schedule (fw_plan_set_stims), copied into each step's
   // kernel arguments
   int32_t n_stims;

@@ -6,6 +6,7 @@
 
 #include "fw_internal.hpp"
 #include "fw_kernels.hpp"
+#include "fw_march3d.hpp"
 
 #ifndef FW_PREC
 #error "FW_PREC must be defined"
@@ -25,6 +26,23 @@ using UReal = UType<FW_MODEL, Real>;  // storage type of u / u_new
 
 template <int NDIM, int ST>
 static void launch(const fw_plan* plan, const KArgs<Real, UReal>& a, hipStream_t s) {
+#if FW_PREC == 1 && FW_MODEL < 5  // the float-potential models
+  if constexpr (NDIM == 3 && ST == FW_STENCIL_ANISO && std::is_same_v<UReal, float>) {
+    // the FP32 19-point step that computes each face and square once
+    // (fw_march3d.hpp), for fields uniform or tabulated along k
+    MarchGeom g;
+    if ((a.field_layout == kFieldsUniform || a.field_layout == kFieldsK) &&
+        march_geom(a.ni, a.nj, a.nk, g)) {
+      const dim3 grid(g.per_xcd * kXcds);
+      if (a.field_layout == kFieldsUniform)
+        hipLaunchKernelGGL((k_step3_march<FW_MODEL, kFieldsUniform>), grid, dim3(kBlock), 0, s,
+                           a, g);
+      else
+        hipLaunchKernelGGL((k_step3_march<FW_MODEL, kFieldsK>), grid, dim3(kBlock), 0, s, a, g);
+      return;
+    }
+  }
+#endif
   if constexpr (ST != FW_STENCIL_WEIGHTS) {
     if (a.field_layout == kFieldsUniform) {
       hipLaunchKernelGGL((k_step<NDIM, ST, FW_MODEL, Real, FW_PREC, kFieldsUniform>),

@@ -286,8 +286,6 @@ class Engine:
         track = _native.fw_track_desc()
         keep = []
         if self.stim_sched:
-            if stim_first:
-                self.apply_stims_at(step0)
             track.stim_end = step0 + n
         if act_tracker is not None:
             act = self.act[act_tracker]
@@ -306,34 +304,26 @@ class Engine:
             track.probe_rows = rows.ctypes.data
         stream = _stream()
         uniform = act_flags is None or (np.all(act_flags == act_flags[0]))
-        graph_ok = use_graph and probes is None and uniform and n >= 8
-        s0 = step0
+        graph_ok = use_graph and probes is None and uniform and n >= 2
+        u_ptr, un_ptr = self.u_dev().data_ptr(), self.u_new_dev().data_ptr()
+        if self.stim_sched and stim_first:
+            # launched last before the steps: no host work between the two
+            check(L.fw_stim_apply_step(self.plan.handle, u_ptr, int(step0), stream),
+                  "fw_stim_apply_step")
         if graph_ok:
-            if self.swapped:  # one plain step restores the canonical orientation
-                self._plain(track, s0, 1, act_flags, 0, stream)
-                s0 += 1
-                n -= 1
-            pairs = n // 2
-            if pairs:
-                sub = _native.fw_track_desc.from_buffer_copy(track)
-                if act_flags is not None:
-                    f1 = np.full(2 * pairs, act_flags[0], dtype=np.uint8)
-                    keep.append(f1)
-                    sub.act_flags = f1.ctypes.data
-                check(L.fw_run_graph(self.plan.handle, self.A.data_ptr(), self.B.data_ptr(),
-                                     self._state_ptrs, s0, 2 * pairs, byref(sub), stream),
-                      "fw_run_graph")
-                s0 += 2 * pairs
-                n -= 2 * pairs
-            if n:
-                self._plain(track, s0, n, act_flags, s0 - step0, stream)
+            # graphs exist for both buffer orientations; an odd n ends with
+            # one plain launch inside fw_run_graph
+            check(L.fw_run_graph(self.plan.handle, u_ptr, un_ptr, self._state_ptrs, step0, n,
+                                 byref(track), stream), "fw_run_graph")
+            if n % 2:
+                self.swapped = not self.swapped
             return
-        self._plain(track, s0, n, act_flags, 0, stream)
+        self._plain(track, step0, n, act_flags, 0, stream)
 
     def prepare_graphs(self, n: int, act_tracker=None, act_on: bool = True) -> None:
-        """Captures the graphs a graph-replayed run_steps of n steps would
-        use (fw_run_graph_prepare), launching nothing; run_steps then only
-        replays them."""
+        """Captures the graphs a graph-replayed run_steps of n steps from the
+        current buffer orientation would use (fw_run_graph_prepare),
+        launching nothing; run_steps then only replays them."""
         pairs = n // 2
         if pairs <= 0:
             return
@@ -344,9 +334,9 @@ class Engine:
             track.act = self.act[act_tracker].data_ptr()
             track.act_thr = float(act_tracker.threshold)
             track.act_flags = flags.ctypes.data
-        check(lib().fw_run_graph_prepare(self.plan.handle, self.A.data_ptr(), self.B.data_ptr(),
-                                         self._state_ptrs, 2 * pairs, byref(track)),
-              "fw_run_graph_prepare")
+        check(lib().fw_run_graph_prepare(self.plan.handle, self.u_dev().data_ptr(),
+                                         self.u_new_dev().data_ptr(), self._state_ptrs,
+                                         2 * pairs, byref(track)), "fw_run_graph_prepare")
 
     def _plain(self, track, step0, n, act_flags, offset, stream) -> None:
         sub = _native.fw_track_desc.from_buffer_copy(track)
