@@ -23,7 +23,8 @@ FIELD_UNIFORM, FIELD_AXIS, FIELD_FULL = 0, 1, 2
 CODE_CONDUCT, CODE_FROZEN, CODE_PROBE = 1, 2, 4
 MAX_STATE, MAX_PARAMS, MAX_FIELDS, MAX_STIMS = 24, 64, 9, 8
 NOALLOC_SCRATCH_BYTES = 4 << 20
-ABI_VERSION = 2
+BATCH_ALT_STATE = 4
+ABI_VERSION = 3
 
 # every symbol include/finitewave_amd.h declares
 EXPORTS = (
@@ -38,7 +39,7 @@ EXPORTS = (
     "fw_batch_set_tissue", "fw_batch_set_active", "fw_batch_release", "fw_batch_block_nodes",
     "fw_batch_variants", "fw_batch_set_variant", "fw_batch_get_variant",
     "fw_batch_run",
-    "fw_batch_stim",
+    "fw_batch_stim", "fw_batch_nonfinite",
 )
 
 

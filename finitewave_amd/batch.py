@@ -213,6 +213,8 @@ class TissueBatch:
                                    ctypes.c_int32, ctypes.c_void_p]
         L.fw_batch_stim.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64,
                                     ctypes.POINTER(_native.fw_stim_desc), ctypes.c_void_p]
+        L.fw_batch_nonfinite.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                         ctypes.c_void_p]
         check(L.fw_batch_create(self.model_id, self.prec, float(self.proto.D_model), dt,
                                 dr ** 2, par, len(params), max_tissues,
                                 ctypes.byref(self.handle)), "fw_batch_create")
@@ -231,6 +233,10 @@ class TissueBatch:
         self.names = self.proto._state_call_order()
         self.f64 = [_native.state_is_f64(self.model_id, self.prec, k)
                     for k in range(len(self.names))]
+        # the multi-step launch shapes (light models) write a second state set
+        self.alt_state = self.model_id <= 4
+        self._flags = None   # pinned host flags of the divergence check
+        self._check = None   # (event, sims) of the check in flight
 
     # ------------------------------------------------------------------ time
     def _t_fold(self, n: int) -> np.ndarray:
@@ -240,10 +246,13 @@ class TissueBatch:
         return self._fold
 
     def footprint(self, shape, horizon_steps) -> int:
-        """Device bytes a tissue of `shape` needs (state, codes, maps, probe)."""
+        """Device bytes a tissue of `shape` needs (state and, for the light
+        models, its second set; codes, maps, probe)."""
         n = int(np.prod(shape))
         rb = 8 if _native.u_is_f64(self.model_id, self.prec) else 4
         st = sum(8 if f else 4 for f in self.f64)
+        if self.alt_state:
+            st *= 2
         return n * (2 * rb + st + 1 + 4) + rb * (horizon_steps + 1)
 
     # ------------------------------------------------------------------ admission
@@ -272,6 +281,7 @@ class TissueBatch:
         state = [torch.full(shape, float(inits[nm]), device=self.dev,
                             dtype=torch.float64 if f else torch.float32)
                  for nm, f in zip(self.names, self.f64)]
+        state_alt = [torch.empty_like(x) for x in state] if self.alt_state else []
         code = torch.from_numpy(tissue.node_codes()).to(self.dev)
         n_steps = run_length(sim.t_max, self.dt, self._t_fold(int(math.ceil(sim.t_max / self.dt)) + 1))
         fold = self._t_fold(n_steps + 1)
@@ -289,6 +299,8 @@ class TissueBatch:
         desc.u, desc.un = u.data_ptr(), un.data_ptr()
         for k, s in enumerate(state):
             desc.state[k] = s.data_ptr()
+        for k, s in enumerate(state_alt):
+            desc.state_alt[k] = s.data_ptr()
         desc.code = code.data_ptr()
         desc.act = act.data_ptr() if act is not None else 0
         desc.probe = probe.data_ptr() if probe is not None else 0
@@ -305,8 +317,8 @@ class TissueBatch:
                                         self._stream()), "fw_batch_set_tissue")
         sim.slot, sim.step0, sim.n_steps = slot, self.step, n_steps
         self.sims[slot] = sim
-        self.buffers[slot] = dict(u=u, un=un, state=state, code=code, act=act, probe=probe,
-                                  shape=shape, desc=desc)
+        self.buffers[slot] = dict(u=u, un=un, state=state, state_alt=state_alt, code=code,
+                                  act=act, probe=probe, shape=shape, desc=desc)
         self.events[slot] = stim_schedule(sim.stims, n_steps, fold)
         self.blocks_dirty = True
         return slot
@@ -345,6 +357,57 @@ class TissueBatch:
         self.free.append(slot)
         self.blocks_dirty = True
         return sim
+
+    @_on_stream
+    def drop(self, slot: int) -> BatchSim:
+        """Removes a tissue without reading its results (a cancelled or
+        diverged request); its buffers are freed once the batch's stream has
+        finished with them."""
+        sim = self.sims.pop(slot)
+        self.stream.synchronize()
+        self.buffers.pop(slot)
+        self.events.pop(slot, None)
+        check(lib().fw_batch_release(self.handle, slot), "fw_batch_release")
+        self.free.append(slot)
+        self.blocks_dirty = True
+        return sim
+
+    @_on_stream
+    def check_nonfinite(self) -> None:
+        """Enqueues a divergence check of every tissue at the current step
+        (fw_batch_nonfinite into pinned host memory); poll_nonfinite reads it
+        once the stream has reached it."""
+        if not self.sims or self._check is not None:
+            return
+        if self.blocks_dirty:
+            self._rebuild_blocks()
+        torch = self.torch
+        if self._flags is None:
+            self._flags = torch.zeros(self.max_tissues, dtype=torch.int32, pin_memory=True)
+        check(lib().fw_batch_nonfinite(self.handle, self.step, self._flags.data_ptr(),
+                                       self._stream()), "fw_batch_nonfinite")
+        ev = torch.cuda.Event()
+        ev.record(self.stream)
+        self._check = (ev, dict(self.sims))
+
+    def poll_nonfinite(self, wait: bool = False) -> list:
+        """Slots found non-finite by the last check once it has completed
+        (tissues that left the batch since are skipped); [] while it runs."""
+        if self._check is None:
+            return []
+        ev, sims = self._check
+        if not wait and not ev.query():
+            return []
+        ev.synchronize()
+        self._check = None
+        flags = self._flags.numpy()
+        return [slot for slot, sim in sims.items()
+                if flags[slot] and self.sims.get(slot) is sim]
+
+    def nonfinite(self) -> list:
+        """Slots whose potential holds a NaN or an infinity now."""
+        self.check_nonfinite()
+        return self.poll_nonfinite(wait=True)
 
     # ------------------------------------------------------------------ stepping
     def _stream(self):
@@ -497,4 +560,5 @@ class _TissueDesc(ctypes.Structure):
                 ("d", ctypes.c_double * 4), ("act_thr", ctypes.c_double),
                 ("stencil", ctypes.c_int32), ("step0", ctypes.c_int32),
                 ("act_start", ctypes.c_int32), ("probe_idx", ctypes.c_int32),
-                ("probe_len", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("probe_len", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("state_alt", ctypes.c_void_p * _native.BATCH_ALT_STATE)]

@@ -140,6 +140,31 @@ __global__ void k_check_codes(const uint8_t* __restrict__ code, int64_t ni, int6
   if (found) atomicOr(bad, found);
 }
 
+// flags[t] |= 1 for every tissue t of the block table whose potential
+// before global step `step` holds a NaN or an infinity: logical block lb
+// covers `per` nodes of tissue btis[lb] from node bstart[lb].
+template <typename U>
+__global__ void k_batch_nonfinite(const fw::BatchTissue* __restrict__ T,
+                                  const int32_t* __restrict__ btis,
+                                  const int32_t* __restrict__ bstart, int32_t per,
+                                  int32_t step, int32_t* flags) {
+  const int32_t t = btis[blockIdx.x], start = bstart[blockIdx.x];
+  const fw::BatchTissue& d = T[t];
+  const int32_t ls = step - d.step0;
+  if (ls < 0) return;
+  const U* __restrict__ u = (const U*)((ls & 1) ? d.un : d.u);
+  const int32_t end = start + per < d.n ? start + per : d.n;
+  bool bad = false;
+  for (int32_t c = start + (int32_t)threadIdx.x; c < end; c += (int32_t)blockDim.x)
+    bad |= !__builtin_isfinite(u[c]);
+  if (bad) atomicOr(flags + t, 1);
+}
+
+__global__ void k_fill_i32(int32_t* p, int32_t n, int32_t v) {
+  const int32_t i = (int32_t)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i < n) p[i] = v;
+}
+
 // the seed mixed once on the host (SplitMix64's output function, below)
 static uint64_t host_splitmix64(uint64_t x) {
   x += 0x9e3779b97f4a7c15ull;
@@ -1010,6 +1035,22 @@ BatchLauncher batch_launcher(int precision, int model) {
   return table[precision][model];
 }
 
+#define FW_DECL_TLAUNCHER(p, m) \
+  int fw_batch_tb_launch_p##p##_m##m(const BatchLaunch&, const TbLaunch&, hipStream_t);
+FW_DECL_TLAUNCHER(0, 0) FW_DECL_TLAUNCHER(0, 1) FW_DECL_TLAUNCHER(0, 2) FW_DECL_TLAUNCHER(0, 3)
+FW_DECL_TLAUNCHER(0, 4) FW_DECL_TLAUNCHER(1, 0) FW_DECL_TLAUNCHER(1, 1) FW_DECL_TLAUNCHER(1, 2)
+FW_DECL_TLAUNCHER(1, 3) FW_DECL_TLAUNCHER(1, 4)
+
+BatchTbLauncher batch_tb_launcher(int precision, int model) {
+  static const BatchTbLauncher table[2][5] = {
+      {fw_batch_tb_launch_p0_m0, fw_batch_tb_launch_p0_m1, fw_batch_tb_launch_p0_m2,
+       fw_batch_tb_launch_p0_m3, fw_batch_tb_launch_p0_m4},
+      {fw_batch_tb_launch_p1_m0, fw_batch_tb_launch_p1_m1, fw_batch_tb_launch_p1_m2,
+       fw_batch_tb_launch_p1_m3, fw_batch_tb_launch_p1_m4}};
+  if (precision < 0 || precision > 1 || model < 0 || model > 4) return nullptr;
+  return table[precision][model];
+}
+
 }  // namespace fw
 
 static BatchLaunch batch_args(const fw_batch* b, int32_t step, const int32_t* in, int32_t* out) {
@@ -1022,13 +1063,110 @@ static BatchLaunch batch_args(const fw_batch* b, int32_t step, const int32_t* in
   L.step_in = in;
   L.step_out = out;
   L.step = step;
-  L.variant = b->variant;
+  L.variant = b->variant & (kBatchVariants - 1);  // the single-step shape
   L.D = b->D;
   L.dt = b->dt;
   L.dr2 = b->dr2;
   L.inv_dr2 = b->inv_dr2;
   L.P = b->P;
   return L;
+}
+
+// The units of the multi-step shape (fw_batch_tb.hpp) for one tissue:
+// column strips of 64 lanes whose own columns are K inside the window
+// except at the tissue's edges, each split into runs of about `rows` rows.
+static void tb_tissue_units(const fw_tissue_desc& d, int32_t slot, int K, int rows,
+                            TbUnit* out, int64_t* n) {
+  const int32_t ni = (int32_t)d.ni, nj = (int32_t)d.nj;
+  const int32_t nseg = (ni + rows - 1) / rows;
+  const int32_t len = (ni + nseg - 1) / nseg;
+  for (int32_t c = 0; c < nj;) {
+    const int32_t w0 = c == 0 ? 0 : c - K;
+    const int32_t c1 = w0 + 64 >= nj ? nj : w0 + 64 - K;
+    for (int32_t r0 = 0; r0 < ni; r0 += len) {
+      const int32_t r1 = r0 + len < ni ? r0 + len : ni;
+      if (out) out[*n] = TbUnit{slot, w0, c | (c1 << 16), r0 | (r1 << 16)};
+      ++*n;
+    }
+    c = c1;
+  }
+}
+
+static void drop_tb_graphs(fw_batch* b) {
+  if (b->tb_graph) (void)hipGraphExecDestroy(b->tb_graph);
+  if (b->tb_graph_long) (void)hipGraphExecDestroy(b->tb_graph_long);
+  b->tb_graph = b->tb_graph_long = nullptr;
+}
+
+// Builds the unit table of the active tissues for the multi-step variant
+// (the 5-point stencil's units first, then the 9-point stencil's).
+static int build_units(fw_batch* b, const int32_t* slots, int32_t n_slots, hipStream_t s) {
+  const int K = batch_tb_k(b->variant), rows = batch_tb_rows(b->variant);
+  const int nst = model_n_state(b->model);
+  int64_t n = 0;
+  for (int32_t q = 0; q < n_slots; ++q) {
+    const fw_tissue_desc& d = b->host[slots[q]];
+    if (d.ni > 65535 || d.nj > 65535) {
+      set_error("fw_batch_set_active: slot %d is %lld x %lld; the multi-step shapes take "
+                "sides up to 65535", slots[q], (long long)d.ni, (long long)d.nj);
+      return FW_E_INVALID;
+    }
+    for (int k = 0; k < nst; ++k)
+      if (!d.state_alt[k]) {
+        set_error("fw_batch_set_active: slot %d has no second state set (state_alt); the "
+                  "multi-step shapes need one", slots[q]);
+        return FW_E_INVALID;
+      }
+    tb_tissue_units(d, slots[q], K, rows, nullptr, &n);
+  }
+  if (n > INT32_MAX / 8) {
+    set_error("fw_batch_set_active: too many units");
+    return FW_E_INVALID;
+  }
+  TbUnit* hu = (TbUnit*)malloc(sizeof(TbUnit) * (size_t)(n > 0 ? n : 1));
+  if (!hu) {
+    set_error("fw_batch_set_active: out of host memory");
+    return FW_E_NOMEM;
+  }
+  int64_t m = 0, n_iso = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int32_t q = 0; q < n_slots; ++q) {
+      const fw_tissue_desc& d = b->host[slots[q]];
+      if ((d.stencil != FW_STENCIL_ISO) != (pass == 1)) continue;
+      tb_tissue_units(d, slots[q], K, rows, hu, &m);
+      if (pass == 0) n_iso = m;
+    }
+  if (n > b->cap_units) {
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      free(hu);
+      set_error("fw_batch_set_active: stream sync failed");
+      return FW_E_HIP;
+    }
+    if (b->units) (void)hipFree(b->units);
+    b->units = nullptr;
+    b->cap_units = 0;
+    drop_tb_graphs(b);  // they captured the old table's address
+    const int64_t cap = n + n / 4 + 64;
+    if (hipMalloc(&b->units, sizeof(TbUnit) * cap) != hipSuccess) {
+      free(hu);
+      b->units = nullptr;
+      set_error("fw_batch_set_active: allocation failed");
+      return FW_E_NOMEM;
+    }
+    b->cap_units = (int32_t)cap;
+  }
+  if (n && (hipMemcpyAsync(b->units, hu, sizeof(TbUnit) * n, hipMemcpyHostToDevice, s) !=
+                hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)) {
+    free(hu);
+    set_error("fw_batch_set_active: copy failed");
+    return FW_E_HIP;
+  }
+  free(hu);
+  b->n_units = (int32_t)n;
+  b->n_units_iso = (int32_t)n_iso;
+  b->tb_variant = b->variant;
+  return FW_OK;
 }
 
 extern "C" {
@@ -1052,6 +1190,7 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   b->precision = precision;
   b->max_tissues = max_tissues;
   b->variant = batch_default_variant(model);
+  b->tb_variant = -1;
   b->active = new (std::nothrow) int32_t[max_tissues];
   b->D = D_model;
   b->dt = dt;
@@ -1064,7 +1203,8 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   b->host = new (std::nothrow) fw_tissue_desc[max_tissues]();
   if (!b->host || !b->active ||
       hipMalloc(&b->table, sizeof(BatchTissue) * max_tissues) != hipSuccess ||
-      hipMalloc(&b->step_words, 4 * sizeof(int32_t)) != hipSuccess) {
+      hipMalloc(&b->step_words, 4 * sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&b->flags, sizeof(int32_t) * max_tissues) != hipSuccess) {
     set_error("fw_batch_create: allocation failed");
     (void)fw_batch_destroy(b);
     return FW_E_NOMEM;
@@ -1084,10 +1224,14 @@ int fw_batch_destroy(fw_batch* b) {
   if (!b) return FW_OK;
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   if (b->graph_long) (void)hipGraphExecDestroy(b->graph_long);
+  if (b->tb_graph) (void)hipGraphExecDestroy(b->tb_graph);
+  if (b->tb_graph_long) (void)hipGraphExecDestroy(b->tb_graph_long);
+  if (b->units) (void)hipFree(b->units);
   if (b->table) (void)hipFree(b->table);
   if (b->btis) (void)hipFree(b->btis);
   if (b->bstart) (void)hipFree(b->bstart);
   if (b->step_words) (void)hipFree(b->step_words);
+  if (b->flags) (void)hipFree(b->flags);
   delete[] b->host;
   delete[] b->active;
   delete b;
@@ -1145,6 +1289,7 @@ int fw_batch_set_tissue(fw_batch* b, int32_t slot, const fw_tissue_desc* d, void
   r.act = d->act;
   r.probe = d->probe;
   for (int k = 0; k < FW_MAX_STATE; ++k) r.state[k] = d->state[k];
+  for (int k = 0; k < FW_BATCH_ALT_STATE; ++k) r.state_alt[k] = d->state_alt[k];
   for (int q = 0; q < 4; ++q) {
     r.dd[q] = d->d[q];
     r.df[q] = (float)d->d[q];
@@ -1152,6 +1297,7 @@ int fw_batch_set_tissue(fw_batch* b, int32_t slot, const fw_tissue_desc* d, void
   r.act_thr = d->act_thr;
   r.act_thr_f = (float)d->act_thr;
   r.n = (int32_t)d->n;
+  r.ni = (int32_t)d->ni;
   r.nj = (int32_t)d->nj;
   r.step0 = d->step0;
   r.act_start = d->act_start;
@@ -1191,11 +1337,20 @@ int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity) {
     return FW_E_INVALID;
   }
   int32_t n = 0;
+  auto add = [&](int32_t v) {
+    if (n < capacity) variants[n] = v;
+    ++n;
+  };
   for (int32_t v = 0; v < kBatchVariants; ++v)
-    if (batch_variant_ok(model, v)) {
-      if (n < capacity) variants[n] = v;
-      ++n;
-    }
+    if (batch_variant_ok(model, v)) add(v);
+  // multi-step shapes over the XCD-contiguous 4-node single-step shape, with
+  // and without the cache hint
+  for (int32_t base : {kBatchVec | kBatchXcd | kBatchNT, kBatchVec | kBatchXcd})
+    for (int32_t k = 0; k < kBatchTbKs; ++k)
+      for (int32_t r = 0; r < kBatchTbRowSizes; ++r) {
+        const int32_t v = batch_tb_variant(base, k, r);
+        if (batch_variant_ok(model, v)) add(v);
+      }
   return n;
 }
 
@@ -1215,8 +1370,10 @@ int fw_batch_set_variant(fw_batch* b, int32_t variant, void* stream) {
   if (variant == b->variant) return FW_OK;
   const int32_t old = b->variant;
   b->variant = variant;
-  if (batch_unroll(old) != batch_unroll(variant) && b->n_active > 0) {
-    // the block table's granularity changes with the nodes per work-item
+  if (b->n_active > 0 && (batch_unroll(old) != batch_unroll(variant) ||
+                          ((old | variant) & kBatchTB))) {
+    // the block table's granularity changes with the nodes per work-item,
+    // the unit table with the multi-step shape
     int rc = fw_batch_set_active(b, b->active, b->n_active, stream);
     if (rc) {
       b->variant = old;
@@ -1298,6 +1455,11 @@ int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void
   }
   free(hb);
   b->nblocks = (int32_t)nblocks;
+  b->tb_variant = -1;
+  if (b->variant & kBatchTB) {
+    const int rc = build_units(b, slots, n_slots, s);
+    if (rc) return rc;
+  }
   if (slots != b->active)
     for (int32_t q = 0; q < n_slots; ++q) b->active[q] = slots[q];
   b->n_active = n_slots;
@@ -1334,14 +1496,12 @@ static int capture_batch(fw_batch* b, BatchLauncher L, int32_t pairs, hipGraphEx
   return FW_OK;
 }
 
-int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph, void* stream) {
-  if (!b || n_steps < 0) {
-    set_error("fw_batch_run: bad arguments");
-    return FW_E_INVALID;
-  }
-  if (b->nblocks == 0 || n_steps == 0) return FW_OK;
+}  // extern "C"
+
+// n single steps of the batch's single-step shape from step0
+static int batch_run_single(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph,
+                            hipStream_t s) {
   BatchLauncher L = batch_launcher(b->precision, b->model);
-  hipStream_t s = as_stream(stream);
   int32_t done = 0;
   if (use_graph && n_steps >= 2) {
     const int32_t pairs = n_steps / 2;
@@ -1378,6 +1538,165 @@ int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph,
   for (int32_t q = done; q < n_steps; ++q) {
     int rc = L(batch_args(b, (int32_t)(step0 + q), nullptr, nullptr), s);
     if (rc) return rc;
+  }
+  return FW_OK;
+}
+
+// One pair of multi-step launches (phase 0 then 1): 2K steps from `step`,
+// or from the step word `in` (the pair ends with it again) in a graph.
+static int tb_pair(fw_batch* b, int32_t step, int32_t* in, int32_t* mid, hipStream_t s) {
+  BatchTbLauncher T = batch_tb_launcher(b->precision, b->model);
+  if (!T) {
+    set_error("fw_batch_run: model %d has no multi-step shape", b->model);
+    return FW_E_UNSUPPORTED;
+  }
+  const int K = batch_tb_k(b->variant);
+  const int32_t n_an = b->n_units - b->n_units_iso;
+  for (int phase = 0; phase < 2; ++phase) {
+    BatchLaunch L = batch_args(b, step + phase * K, phase ? mid : in, phase ? in : mid);
+    L.variant = b->variant;
+    for (int an = 0; an < 2; ++an) {
+      const int32_t nu = an ? n_an : b->n_units_iso;
+      if (nu == 0) continue;
+      TbLaunch t;
+      t.units = static_cast<const TbUnit*>(b->units) + (an ? b->n_units_iso : 0);
+      t.n_units = (uint32_t)nu;
+      t.nblocks = (uint32_t)((nu + kTbWaves - 1) / kTbWaves);
+      t.per_xcd = (t.nblocks + kXcds - 1) / kXcds;
+      t.phase = phase;
+      t.aniso = an;
+      const int rc = T(L, t, s);
+      if (rc) return rc;
+    }
+  }
+  return FW_OK;
+}
+
+static int capture_tb(fw_batch* b, int32_t pairs, hipGraphExec_t* out) {
+  hipStream_t cap;
+  if (hipStreamCreateWithFlags(&cap, hipStreamNonBlocking) != hipSuccess) {
+    set_error("fw_batch_run: stream creation failed");
+    return FW_E_HIP;
+  }
+  hipGraph_t g = nullptr;
+  int rc = FW_OK;
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    for (int32_t q = 0; q < pairs && !rc; ++q)
+      rc = tb_pair(b, 0, b->step_words, b->step_words + 1, cap);
+    e = hipStreamEndCapture(cap, &g);
+  }
+  hipGraphExec_t ex = nullptr;
+  if (e == hipSuccess && !rc && g) e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  if (g) (void)hipGraphDestroy(g);
+  if (e == hipSuccess && !rc && ex) e = hipGraphUpload(ex, cap);
+  if (e == hipSuccess && !rc) e = hipStreamSynchronize(cap);
+  (void)hipStreamDestroy(cap);
+  if (e != hipSuccess || rc || !ex) {
+    if (ex) (void)hipGraphExecDestroy(ex);
+    set_error("fw_batch_run: capture failed (%s)", hipGetErrorString(e));
+    return rc ? rc : FW_E_HIP;
+  }
+  *out = ex;
+  return FW_OK;
+}
+
+// multi-step pairs per long graph
+constexpr int32_t kBatchTbLongPairs = 8;
+
+extern "C" {
+
+int fw_batch_run(fw_batch* b, int64_t step0, int32_t n_steps, int32_t use_graph, void* stream) {
+  if (!b || n_steps < 0) {
+    set_error("fw_batch_run: bad arguments");
+    return FW_E_INVALID;
+  }
+  if (b->nblocks == 0 || n_steps == 0) return FW_OK;
+  hipStream_t s = as_stream(stream);
+  if (!(b->variant & kBatchTB)) return batch_run_single(b, step0, n_steps, use_graph, s);
+  if (b->tb_variant != b->variant) {
+    set_error("fw_batch_run: the unit table was not built for this launch shape");
+    return FW_E_INVALID;
+  }
+  // single steps until every active tissue has started
+  int64_t lead = 0;
+  for (int32_t q = 0; q < b->n_active; ++q) {
+    const int64_t d = (int64_t)b->host[b->active[q]].step0 - step0;
+    if (d > lead) lead = d;
+  }
+  if (lead > n_steps) lead = n_steps;
+  if (lead > 0) {
+    const int rc = batch_run_single(b, step0, (int32_t)lead, use_graph, s);
+    if (rc) return rc;
+  }
+  const int32_t K2 = 2 * batch_tb_k(b->variant);
+  const int64_t t0 = step0 + lead;
+  const int32_t pairs = (int32_t)((n_steps - lead) / K2);
+  if (pairs > 0 && use_graph) {
+    if (b->tb_graph_units != b->n_units || b->tb_graph_iso != b->n_units_iso ||
+        b->tb_graph_variant != b->variant) {
+      drop_tb_graphs(b);
+      b->tb_graph_units = b->n_units;
+      b->tb_graph_iso = b->n_units_iso;
+      b->tb_graph_variant = b->variant;
+    }
+    if (!b->tb_graph) {
+      const int rc = capture_tb(b, 1, &b->tb_graph);
+      if (rc) return rc;
+    }
+    if (pairs >= kBatchTbLongPairs && !b->tb_graph_long) {
+      const int rc = capture_tb(b, kBatchTbLongPairs, &b->tb_graph_long);
+      if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_set_word, dim3(1), dim3(1), 0, s, b->step_words, (int32_t)t0);
+    int32_t q = 0;
+    for (; q + kBatchTbLongPairs <= pairs; q += kBatchTbLongPairs)
+      if (hipGraphLaunch(b->tb_graph_long, s) != hipSuccess) {
+        set_error("fw_batch_run: graph launch failed");
+        return FW_E_HIP;
+      }
+    for (; q < pairs; ++q)
+      if (hipGraphLaunch(b->tb_graph, s) != hipSuccess) {
+        set_error("fw_batch_run: graph launch failed");
+        return FW_E_HIP;
+      }
+  } else {
+    for (int32_t q = 0; q < pairs; ++q) {
+      const int rc = tb_pair(b, (int32_t)(t0 + (int64_t)q * K2), nullptr, nullptr, s);
+      if (rc) return rc;
+    }
+  }
+  const int64_t done = lead + (int64_t)pairs * K2;
+  if (done < n_steps)
+    return batch_run_single(b, step0 + done, (int32_t)(n_steps - done), use_graph, s);
+  return FW_OK;
+}
+
+int fw_batch_nonfinite(fw_batch* b, int64_t step, int32_t* flags, void* stream) {
+  if (!b || !flags) {
+    set_error("fw_batch_nonfinite: bad arguments");
+    return FW_E_INVALID;
+  }
+  hipStream_t s = as_stream(stream);
+  const int32_t nt = b->max_tissues;
+  hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, b->flags,
+                     nt, 0);
+  if (b->nblocks > 0) {
+    const int32_t per = batch_unroll(b->variant) * kBlock;
+    if (u_is_f64(b->model, b->precision))
+      hipLaunchKernelGGL(k_batch_nonfinite<double>, dim3((unsigned)b->nblocks), dim3(256), 0, s,
+                         static_cast<const BatchTissue*>(b->table), b->btis, b->bstart, per,
+                         (int32_t)step, b->flags);
+    else
+      hipLaunchKernelGGL(k_batch_nonfinite<float>, dim3((unsigned)b->nblocks), dim3(256), 0, s,
+                         static_cast<const BatchTissue*>(b->table), b->btis, b->bstart, per,
+                         (int32_t)step, b->flags);
+  }
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(flags, b->flags, sizeof(int32_t) * nt, hipMemcpyDeviceToHost, s) !=
+          hipSuccess) {
+    set_error("fw_batch_nonfinite: check failed");
+    return FW_E_HIP;
   }
   return FW_OK;
 }

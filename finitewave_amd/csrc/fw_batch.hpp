@@ -44,11 +44,33 @@ struct BatchTissue {
   int32_t* act;
   void* probe;
   void* state[FW_MAX_STATE];
+  void* state_alt[FW_BATCH_ALT_STATE];  // the multi-step shapes' second state set
   double dd[4];      // half-step fields (FP64 weights)
   double act_thr;
   float df[4];       // the same fields rounded to float (FP32 flux form)
   float act_thr_f;
-  int32_t n, nj, step0, act_start, probe_idx, probe_len, stencil;
+  int32_t n, ni, nj, step0, act_start, probe_idx, probe_len, stencil;
+};
+
+// A unit of the multi-step launch shape (fw_batch_tb.hpp): tissue, first
+// column of its 64-lane window, own columns [c0, c1) and own rows [r0, r1)
+// (16 bits each).
+struct alignas(16) TbUnit {
+  int32_t tissue;
+  int32_t w0;
+  int32_t cols;  // c0 | c1 << 16
+  int32_t rows;  // r0 | r1 << 16
+};
+
+constexpr int kTbWaves = kBlock / 64;  // units per block
+
+struct TbLaunch {
+  const TbUnit* __restrict__ units;  // this launch's units (of one stencil)
+  uint32_t n_units;
+  uint32_t nblocks;  // launched blocks hold kBlock / 64 units each
+  uint32_t per_xcd;  // blocks per XCD of the XCD-contiguous map
+  int32_t phase;     // 0: state -> state_alt, 1: state_alt -> state
+  int32_t aniso;     // the units' tissues use the 9-point stencil
 };
 
 // R: arithmetic type of the ionic update; U: storage type of u / u_new
@@ -130,10 +152,31 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
 //   bit 2: state and new potential read and written with the non-temporal
 //          hint (faster once a step's data is well beyond the Infinity Cache).
 // A block covers batch_unroll(variant) * kBlock consecutive nodes of a tissue.
+//   bit 3 (light models): K = batch_tb_k(variant) steps per launch, one
+//          wavefront per 64 columns x batch_tb_rows(variant) rows of a tissue
+//          (fw_batch_tb.hpp), in pairs of launches; bits 0-2 then give the
+//          single-step shape that runs the steps a whole pair does not cover,
+//          and bit 2 also the multi-step launch's cache hint.
 constexpr int kBatchVec = 1, kBatchXcd = 2, kBatchNT = 4, kBatchVariants = 8;
+constexpr int kBatchTB = 8;
+constexpr int kBatchTbKs = 2, kBatchTbRowSizes = 2;  // K in {2, 4}, rows in {64, 128}
+
+__host__ __device__ constexpr int batch_tb_k(int variant) { return 2 << ((variant >> 4) & 3); }
+__host__ __device__ constexpr int batch_tb_rows(int variant) { return 64 << ((variant >> 6) & 3); }
+__host__ __device__ constexpr int batch_tb_variant(int base, int kidx, int ridx) {
+  return base | kBatchTB | (kidx << 4) | (ridx << 6);
+}
+
+__host__ __device__ constexpr bool batch_base_ok(int model, int base) {
+  return base >= 0 && base < kBatchVariants && (model <= 4 || (base & kBatchVec) == 0);
+}
 
 __host__ __device__ constexpr bool batch_variant_ok(int model, int variant) {
-  return variant >= 0 && variant < kBatchVariants && (model <= 4 || (variant & kBatchVec) == 0);
+  return (variant & kBatchTB)
+             ? (model <= 4 && variant >= 0 && (variant >> 8) == 0 &&
+                ((variant >> 4) & 3) < kBatchTbKs && ((variant >> 6) & 3) < kBatchTbRowSizes &&
+                batch_base_ok(model, variant & 7))
+             : batch_base_ok(model, variant);
 }
 
 __host__ __device__ constexpr int batch_default_variant(int model) {

@@ -133,6 +133,12 @@ struct BatchLaunch {
 typedef int (*BatchLauncher)(const BatchLaunch& L, hipStream_t stream);
 BatchLauncher batch_launcher(int precision, int model);
 
+// the multi-step shape of the light models (null for the other models):
+// L.variant selects K and the cache hint, T the units and the phase
+struct TbLaunch;
+typedef int (*BatchTbLauncher)(const BatchLaunch& L, const TbLaunch& T, hipStream_t stream);
+BatchTbLauncher batch_tb_launcher(int precision, int model);
+
 }  // namespace fw
 
 struct fw_batch {
@@ -144,6 +150,7 @@ struct fw_batch {
   int32_t* bstart;
   int32_t nblocks, cap_blocks;
   int32_t* step_words;    // device int32[4]: graph step words [0..1], border flag [2]
+  int32_t* flags;         // device int32[max_tissues]: per-tissue flags of fw_batch_nonfinite
   hipGraphExec_t graph;       // two steps
   hipGraphExec_t graph_long;  // 2 * kGraphLongPairs steps, for long runs
   int32_t graph_nblocks, graph_variant;
@@ -151,4 +158,10 @@ struct fw_batch {
   int32_t* active;        // host: the occupied slots listed by fw_batch_set_active
   int32_t n_active;
   fw_tissue_desc* host;   // host mirror of the table
+  // multi-step shape: unit table of the active tissues (built for tb_variant)
+  void* units;            // device fw::TbUnit[cap_units]
+  int32_t n_units, n_units_iso, cap_units, tb_variant;
+  hipGraphExec_t tb_graph;       // one pair of multi-step launches
+  hipGraphExec_t tb_graph_long;  // kBatchTbLongPairs pairs
+  int32_t tb_graph_units, tb_graph_iso, tb_graph_variant;
 };

@@ -8,6 +8,13 @@ has one ``TissueBatch``; the loop advances the batches in turn by short
 chunks of steps, retires finished tissues (their activation map, probe trace
 and APD90 are read back) and records each request's latency from arrival to
 results.
+
+A request ends in one of three states: ``done`` (results read back),
+``cancelled`` (``Server.cancel(rid)`` or its ``deadline_s`` passed: it
+leaves the queue, or its tissue leaves its batch at the next launch
+boundary) or ``diverged`` (its potential held a NaN or an infinity at a
+check, every ``check_every`` chunks, one launch per batch: the tissue is
+dropped and the rest of the batch runs on).
 """
 from __future__ import annotations
 
@@ -29,6 +36,9 @@ class Request:
     footprint: int = 0
     admitted: float | None = None
     finished: float | None = None
+    deadline_s: float | None = None   # cancelled when unfinished this long after arrival
+    status: str = "pending"           # pending, active, done, cancelled, diverged
+    error: str | None = None
 
     @property
     def latency(self):
@@ -38,7 +48,7 @@ class Request:
 class Server:
     def __init__(self, model_classes: dict, precision: str = "fp32", dt: float = 0.01,
                  dr: float = 0.25, hbm_budget: int | None = None, max_tissues: int = 256,
-                 chunk_steps: int = 200, use_graph: bool = True):
+                 chunk_steps: int = 200, use_graph: bool = True, check_every: int = 1):
         import torch
         self.batches = {k: TissueBatch(cls, precision=precision, dt=dt, dr=dr,
                                        max_tissues=max_tissues, use_graph=use_graph,
@@ -51,8 +61,17 @@ class Server:
         self.in_use = 0
         self.chunk_steps = chunk_steps
         self.completed: list[Request] = []
+        self.cancelled: list[Request] = []
+        self.diverged: list[Request] = []
+        self.check_every = check_every
+        self._cancel: set = set()
         self.peak_in_use = 0
         self.max_concurrent = 0
+
+    def cancel(self, rid: int) -> None:
+        """Cancels request `rid`: it is not admitted, or leaves its batch at
+        the next launch boundary; results are not read."""
+        self._cancel.add(rid)
 
     def footprint(self, req: Request) -> int:
         b = self.batches[req.model]
@@ -71,8 +90,33 @@ class Server:
                 raise ValueError(f"request {r.rid} needs {r.footprint} B, above the budget")
         active: dict = {}
         t0 = time.perf_counter()
+        chunks = 0
+
+        def expired(r, now):
+            return r.rid in self._cancel or (r.deadline_s is not None and
+                                             now - r.arrival > r.deadline_s)
+
+        def end(r, status, error=None):
+            r.finished = time.perf_counter() - t0
+            r.status, r.error = status, error
+            (self.cancelled if status == "cancelled" else self.diverged).append(r)
+            if log:
+                log(r)
+
         while pending or active:
             now = time.perf_counter() - t0
+            # cancellations: queued requests leave the queue, running ones
+            # their batch
+            for r in [r for r in pending if (not realtime or r.arrival <= now) and
+                      expired(r, now)]:
+                pending.remove(r)
+                end(r, "cancelled")
+            for key, r in list(active.items()):
+                if expired(r, now):
+                    self.batches[r.model].drop(r.sim.slot)
+                    del active[key]
+                    self.in_use -= r.footprint
+                    end(r, "cancelled")
             # admission
             while pending and (not realtime or pending[0].arrival <= now):
                 r = pending[0]
@@ -81,6 +125,7 @@ class Server:
                     break
                 pending.popleft()
                 b.admit(r.sim)
+                r.status = "active"
                 r.admitted = time.perf_counter() - t0
                 active[id(r.sim)] = r
                 self.in_use += r.footprint
@@ -91,25 +136,38 @@ class Server:
                     time.sleep(max(0.0, min(0.01, pending[0].arrival - now)))
                 continue
             # one chunk on every batch with work
+            chunks += 1
             for b in self.batches.values():
                 if not b.sims:
                     continue
                 for sim in b.advance(self.chunk_steps):
                     r = active.pop(id(sim))
                     r.finished = time.perf_counter() - t0
+                    r.status = "done"
                     self.in_use -= r.footprint
                     self.completed.append(r)
                     if log:
                         log(r)
+                # divergence: a check enqueued behind this chunk, read when done
+                for slot in b.poll_nonfinite():
+                    sim = b.drop(slot)
+                    r = active.pop(id(sim))
+                    self.in_use -= r.footprint
+                    end(r, "diverged", f"non-finite potential by global step {b.step}")
+                if self.check_every and chunks % self.check_every == 0:
+                    b.check_nonfinite()
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        lat = np.array([r.latency for r in self.completed])
-        first = min(r.arrival for r in self.completed)
-        last = max(r.finished for r in self.completed)
-        return dict(completed=len(self.completed), wall_s=wall,
-                    sims_per_s=len(self.completed) / max(1e-9, last - first),
-                    p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)),
-                    peak_hbm_bytes=self.peak_in_use, max_concurrent=self.max_concurrent)
+        out = dict(completed=len(self.completed), cancelled=len(self.cancelled),
+                   diverged=len(self.diverged), wall_s=wall, peak_hbm_bytes=self.peak_in_use,
+                   max_concurrent=self.max_concurrent)
+        if self.completed:
+            lat = np.array([r.latency for r in self.completed])
+            first = min(r.arrival for r in self.completed)
+            last = max(r.finished for r in self.completed)
+            out.update(sims_per_s=len(self.completed) / max(1e-9, last - first),
+                       p50_s=float(np.percentile(lat, 50)), p99_s=float(np.percentile(lat, 99)))
+        return out
 
     def close(self):
         for b in self.batches.values():

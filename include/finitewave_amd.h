@@ -69,7 +69,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 2
+#define FW_ABI_VERSION 3
 
 /* status codes */
 #define FW_OK 0
@@ -350,6 +350,7 @@ int fw_fibrosis(int8_t* mesh, int32_t ndim, const int64_t* shape, const int64_t*
  * tissue = global step - step0; its u is in `u` at even local steps and in
  * `un` at odd ones.  act records local steps, probe[ls] = u at probe_idx
  * before local step ls, in the batch's precision (double or float). */
+#define FW_BATCH_ALT_STATE 4
 typedef struct fw_tissue_desc {
   void* u;
   void* un;
@@ -366,6 +367,11 @@ typedef struct fw_tissue_desc {
   int32_t probe_idx; /* local flat node, < 0 none */
   int32_t probe_len; /* entries of probe; local steps beyond it are not sampled */
   int32_t reserved;
+  /* models 0-4: a second set of the state arrays (same types and sizes),
+   * which the multi-step launch shapes write between their two launches of
+   * a pair; null entries make those shapes unavailable while the tissue is
+   * active.  Their contents need no initialisation. */
+  void* state_alt[FW_BATCH_ALT_STATE];
 } fw_tissue_desc;
 
 typedef struct fw_batch fw_batch;
@@ -394,18 +400,30 @@ int fw_batch_block_nodes(const fw_batch* batch);
 /* Launch-shape variants of the batched step for a model (all give the same
  * results bit for bit): bit 0 = 4 consecutive nodes per work-item (models
  * 0-4), bit 1 = XCD-contiguous block map, bit 2 = state and new potential
- * accessed with the non-temporal cache hint.  Writes up to `capacity` of them
- * to `variants` and returns how many there are. */
+ * accessed with the non-temporal cache hint.  Bit 3 (models 0-4) = K steps
+ * per launch, K = 2 << bits 4-5, each wavefront marching down 64 columns by
+ * 64 << bits 6-7 rows of a tissue; steps are run as pairs of such launches
+ * (2K steps), the rest with the shape of bits 0-2.  Writes up to `capacity`
+ * of them to `variants` and returns how many there are. */
 int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity);
 /* Selects the launch shape of fw_batch_run (rebuilds the block table when the
  * nodes per work-item change); returns FW_E_INVALID for a variant the model
  * does not have. */
 int fw_batch_set_variant(fw_batch* batch, int32_t variant, void* stream);
 int fw_batch_get_variant(const fw_batch* batch);
-/* n_steps global steps from step0; use_graph replays a captured two-step
- * graph (recaptured when the block count changes). */
+/* n_steps global steps from step0; use_graph replays captured graphs
+ * (recaptured when the block table or launch shape changes).  No stimulus
+ * or admission may fall inside the run: the host applies them between
+ * runs. */
 int fw_batch_run(fw_batch* batch, int64_t step0, int32_t n_steps, int32_t use_graph,
                  void* stream);
+/* Divergence check: flags[slot] (host, max_tissues entries) = 1 for every
+ * active tissue whose potential before global step `step` holds a NaN or an
+ * infinity, 0 otherwise.  One launch over the block table, reading each
+ * tissue's current potential once; the copy to `flags` is enqueued on
+ * `stream` (asynchronous for pinned host memory): read flags after the
+ * stream has reached it. */
+int fw_batch_nonfinite(fw_batch* batch, int64_t step, int32_t* flags, void* stream);
 /* Applies a stimulus to slot's potential as it stands before global step
  * `step` (box in the tissue's own coordinates, third axis ignored). */
 int fw_batch_stim(fw_batch* batch, int32_t slot, int64_t step, const fw_stim_desc* stim,

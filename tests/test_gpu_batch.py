@@ -77,10 +77,24 @@ CLS = {"aliev_panfilov": fw.AlievPanfilov2D, "tp06": fw.TP062D,
        "mitchell_schaeffer": fw.MitchellSchaeffer2D}
 
 
+def _first_difference(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.shape != b.shape:
+        return f"shapes {a.shape} vs {b.shape}"
+    bad = np.argwhere(a != b)
+    if len(bad) == 0:
+        return "equal"
+    k = tuple(bad[0])
+    return (f"{len(bad)} of {a.size} differ; first at {k}: {a[k]!r} vs {b[k]!r}; "
+            f"max |diff| {np.max(np.abs(a.astype(np.float64) - b.astype(np.float64))):.3e}")
+
+
 def _check_same(res, ref):
-    assert np.array_equal(res["act_t"], ref["act_t"])
-    assert np.array_equal(res["trace"], ref["trace"].astype(np.float64))
-    assert np.array_equal(res["u"], ref["u"])
+    for key in ("u", "act_t", "trace"):
+        a, b = np.asarray(res[key]), np.asarray(ref[key])
+        if key == "trace":
+            b = b.astype(np.float64)
+        assert np.array_equal(a, b), f"{key}: {_first_difference(a, b)}"
 
 
 @pytest.mark.gpu
@@ -165,6 +179,82 @@ def test_serving_admission_respects_hbm_budget(gpu, fit):
     assert stats["completed"] == 5
     assert stats["peak_hbm_bytes"] <= srv.hbm_budget
     assert stats["max_concurrent"] == fit
+
+
+def _ap_request(rid, seed, t_max=3.0, conductivity=1.0):
+    t = fibrotic_tissue(30, 0.05, seed)
+    t.conductivity = conductivity
+    sim = BatchSim(tissue=t, t_max=t_max, stims=[fw.StimVoltageCoord2D(0, 1.0, 0, 32, 0, 4)],
+                   probe=(16, 16), act_threshold=0.5)
+    return Request(rid=rid, arrival=0.0, model="ap", sim=sim)
+
+
+@pytest.mark.gpu
+def test_serving_cancels_queued_and_running_requests(gpu):
+    """A request cancelled while queued is never admitted, one cancelled
+    while running leaves its batch at the next launch boundary, one whose
+    deadline passes is cancelled; the others' results equal their runs
+    alone."""
+    classes = {"ap": fw.AlievPanfilov2D}
+    reqs = [_ap_request(k, 300 + k) for k in range(5)]
+    reqs[3].deadline_s = 0.0            # expires at the first look
+    srv = Server(classes, precision="fp32", max_tissues=2, chunk_steps=40,
+                 hbm_budget=1 << 30)
+    srv.cancel(4)                        # queued behind the two slots
+    seen = []
+
+    def log(r):
+        seen.append((r.rid, r.status))
+
+    # cancel request 1 once it is running: after the first chunk
+    orig = srv.batches["ap"].advance
+
+    def advance(n=None):
+        out = orig(n)
+        if srv.batches["ap"].step >= 40:
+            srv.cancel(1)
+        return out
+    srv.batches["ap"].advance = advance
+    stats = srv.serve(reqs, realtime=False, log=log)
+    srv.close()
+    status = {r.rid: r.status for r in reqs}
+    assert status == {0: "done", 1: "cancelled", 2: "done", 3: "cancelled", 4: "cancelled"}
+    assert reqs[1].admitted is not None and reqs[4].admitted is None
+    assert reqs[1].sim.result is None and reqs[4].sim.result is None
+    assert stats["completed"] == 2 and stats["cancelled"] == 3 and stats["diverged"] == 0
+    assert sorted(seen) == sorted(status.items())
+    for k in (0, 2):
+        alone = Server(classes, precision="fp32", max_tissues=2, chunk_steps=40,
+                       hbm_budget=1 << 30)
+        r = _ap_request(k, 300 + k)
+        alone.serve([r], realtime=False)
+        alone.close()
+        _check_same(reqs[k].sim.result, r.sim.result)
+
+
+@pytest.mark.gpu
+def test_serving_drops_a_diverging_request(gpu):
+    """A tissue whose conductivity makes the explicit step unstable
+    (D dt / dr^2 far above 1/4) reaches non-finite potentials: the divergence
+    check drops it as 'diverged' and the request sharing its batch finishes
+    with the results of its run alone."""
+    classes = {"ap": fw.AlievPanfilov2D}
+    good, bad = _ap_request(0, 400, t_max=4.0), _ap_request(1, 401, t_max=4.0,
+                                                            conductivity=200.0)
+    srv = Server(classes, precision="fp32", max_tissues=2, chunk_steps=25,
+                 hbm_budget=1 << 30)
+    stats = srv.serve([good, bad], realtime=False)
+    srv.close()
+    assert bad.status == "diverged" and "non-finite" in bad.error
+    assert bad.sim.result is None
+    assert good.status == "done"
+    assert stats["completed"] == 1 and stats["diverged"] == 1
+    alone = Server(classes, precision="fp32", max_tissues=2, chunk_steps=25,
+                   hbm_budget=1 << 30)
+    r = _ap_request(0, 400, t_max=4.0)
+    alone.serve([r], realtime=False)
+    alone.close()
+    _check_same(good.sim.result, r.sim.result)
 
 
 @pytest.mark.gpu
@@ -319,3 +409,72 @@ def test_autotuner_chooses_again_when_the_batch_grows(gpu, tmp_path, monkeypatch
     b.close()
     for s in sims:
         _check_same(s.result, _single(fw.MitchellSchaeffer2D, "fp32", s))
+
+
+def _wide_sims(kind, seed):
+    """Tissues wider than one 64-column strip and taller than one run of
+    rows, so that the multi-step shapes' halos and run boundaries fall
+    inside them; second stimuli at steps that split the pairs of launches."""
+    rng = np.random.default_rng(seed)
+    shapes = [(150, 93), (71, 200), (64, 66), (129, 130)]
+    out = []
+    for k, (ni, nj) in enumerate(shapes):
+        t = fw.CardiacTissue2D([ni, nj])
+        mesh = (1 + (rng.random((ni, nj)) <= 0.15)).astype(np.int8)
+        mesh[0], mesh[-1], mesh[:, 0], mesh[:, -1] = 0, 0, 0, 0
+        t.mesh = mesh
+        if kind == "aniso" and k % 2 == 0:
+            a = np.deg2rad(25.0 + 40 * k)
+            f = np.zeros((ni, nj, 2))
+            f[..., 0], f[..., 1] = np.cos(a), np.sin(a)
+            t.fibers = f
+        stims = [fw.StimVoltageCoord2D(0, 1.0, 0, ni, 0, 5)]
+        stims.append(fw.StimVoltageCoord2D(0.37 + 0.11 * k, 1.0, ni // 2, ni, 0, nj // 2))
+        probe = (ni // 2, int(np.flatnonzero(mesh[ni // 2] == 1)[-1]))
+        out.append(BatchSim(tissue=t, t_max=4.0 + 0.53 * k, stims=stims, probe=probe,
+                            act_threshold=0.5, act_start=0.21 * k, apd_threshold=0.5))
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("model,prec,kind", [("mitchell_schaeffer", "fp32", "iso"),
+                                             ("mitchell_schaeffer", "fp32", "aniso"),
+                                             ("aliev_panfilov", "fp64", "aniso"),
+                                             ("bueno_orovio", "fp32", "aniso")])
+def test_multi_step_shapes_on_wide_tissues(gpu, model, prec, kind):
+    """Every multi-step launch shape (K steps per launch, tissues split into
+    64-column strips and runs of rows with K-node halos) gives the single-step
+    shape's results bit for bit on tissues spanning several strips and runs,
+    admitted at different steps, with stimuli between pairs of launches."""
+    cls = {"mitchell_schaeffer": fw.MitchellSchaeffer2D, "aliev_panfilov": fw.AlievPanfilov2D,
+           "bueno_orovio": fw.BuenoOrovio2D}[model]
+    probe = TissueBatch(cls, precision=prec, max_tissues=2)
+    variants = [v for v in probe.variants() if v & 8]
+    probe.close()
+    assert variants
+    results = {}
+    for v in [0] + variants:
+        sims = _wide_sims(kind, seed=5)
+        b = TissueBatch(cls, precision=prec, max_tissues=4, autotune=False, chunk_cap=77)
+        b.set_variant(v)
+        b.admit(sims[0])
+        b.admit(sims[1])
+        b.advance(45)
+        b.admit(sims[2])
+        b.advance(19)
+        b.admit(sims[3])
+        b.run_all()
+        assert b.variant == v
+        b.close()
+        results[v] = [s.result for s in sims]
+        print(f"variant {v}: done", flush=True)
+    ref = results[0]
+    assert any(np.any(r["act_t"] >= 0) for r in ref)
+    bad = []
+    for v in variants:
+        for k, (a, r) in enumerate(zip(results[v], ref)):
+            try:
+                _check_same(a, r)
+            except AssertionError as e:
+                bad.append(f"variant {v} tissue {k}: {e}")
+    assert not bad, "\n".join(bad)

@@ -396,12 +396,17 @@ def test_batch_launch_variants_per_model():
     import ctypes
     L = _native.lib()
     L.fw_batch_variants.argtypes = [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int32]
-    for model, expect in ((2, list(range(8))), (6, [0, 2, 4, 6]), (7, [0, 2, 4, 6])):
-        arr = (ctypes.c_int32 * 8)()
-        n = L.fw_batch_variants(model, arr, 8)
+    # light models: the 8 single-step shapes, then the multi-step ones (bit 3)
+    # for K = 2, 4 (bits 4-5) and runs of 64, 128 rows (bits 6-7) over the
+    # 4-node XCD-contiguous shape with and without the cache hint
+    multi = [base | 8 | (k << 4) | (r << 6) for base in (7, 3) for k in (0, 1) for r in (0, 1)]
+    for model, expect in ((2, list(range(8)) + multi), (0, list(range(8)) + multi),
+                          (6, [0, 2, 4, 6]), (7, [0, 2, 4, 6])):
+        arr = (ctypes.c_int32 * 32)()
+        n = L.fw_batch_variants(model, arr, 32)
         assert list(arr[:n]) == expect
     arr = (ctypes.c_int32 * 2)()
-    assert L.fw_batch_variants(2, arr, 2) == 8 and list(arr) == [0, 1]  # count beyond capacity
+    assert L.fw_batch_variants(2, arr, 2) == 16 and list(arr) == [0, 1]  # count beyond capacity
     assert L.fw_batch_variants(99, None, 0) == -1
 
 
