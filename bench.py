@@ -23,8 +23,10 @@ Beside the headline the line carries the other configurations' legs: the
 512-tissue Mitchell-Schaeffer restitution ensemble (`ensemble`), the
 Poisson-served TP06/Aliev-Panfilov mix (`serving`, with the same requests
 run one after another on the plain FP64 path), the reference's own CPU case
-(`c1`: Aliev-Panfilov 102x102 FP64) and the 3D slab (`c3`: Aliev-Panfilov
-258^3, 19-point stencil, layered fibres, fibrosis, FP32).
+(`c1`: Aliev-Panfilov 102x102 FP64), the headline tissue's S1-S2 spiral
+over 100,000 steps in FP64 and FP32 compared by statistics (`c2_spiral`)
+and the 3D slab (`c3`: Aliev-Panfilov 258^3, 19-point stencil, layered
+fibres, fibrosis, FP32).
 """
 from __future__ import annotations
 
@@ -62,10 +64,12 @@ B_TRACK = 4
 def profiled_traffic(key: str):
     """HBM bytes per launch of a kernel as measured with rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters (scaled by a device copy of known size)
-    and recorded in profiles/*.md as `traffic_bytes_per_launch <key>: <B>`;
+    and recorded in profiles/*.md as `traffic_bytes_per_launch <key>: <B>` (or
+    `traffic_bytes_per_step`, for a launch that advances several steps);
     (None, None) when no summary holds it."""
     for p in sorted((ROOT / "profiles").glob("*.md")):
-        m = re.search(rf"^traffic_bytes_per_launch {key}: (\d+)\s*$", p.read_text(), re.M)
+        m = re.search(rf"^traffic_bytes_per_(?:launch|step) {key}: (\d+)\s*$", p.read_text(),
+                      re.M)
         if m:
             return int(m.group(1)), f"profiles/{p.name}"
     return None, None
@@ -199,6 +203,110 @@ def c1_leg():
                          "planar stimulus, activation-time tracker",
                 us_per_step=round(us, 3), gcell=round(10_000 / us / 1e3, 3),
                 run_1000_steps_ms=round(min(walls) * 1e3, 2), finite=finite)
+
+
+def _tip_track(df):
+    """Per tracked time, the centroid of the spiral tips found then, in
+    nodes: {time: (x, y)}."""
+    out = {}
+    if df is None or len(df) == 0:
+        return out
+    for t, g in df.groupby("time"):
+        out[round(float(t), 6)] = (float(g["x"].mean()), float(g["y"].mean()))
+    return out
+
+
+def _c2_spiral_run(prec: str, s2_time: float, t_max: float, cells):
+    import torch
+    import finitewave_amd as fw
+    n = N_SIDE + 2
+    model, act = build_model(prec)
+    model.t_max = t_max
+    model.stim_sequence.add_stim(fw.StimCurrentCoord2D(s2_time, 100, 1.0, n // 2, n, 0, n // 2))
+    act.threshold = -40
+    tips = fw.SpiralWaveCore2DTracker()
+    tips.threshold = -40
+    tips.start_time = s2_time + 10
+    tips.step = 100
+    per = fw.Period2DTracker()
+    per.cell_ind = cells
+    per.threshold = -40
+    per.step = 10
+    model.tracker_sequence.add_tracker(tips)
+    model.tracker_sequence.add_tracker(per)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    model.run()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    periods = [np.asarray(p, dtype=np.float64) for p in per.output]
+    out = tips.output
+    print(f"# c2 spiral S2 {s2_time} {prec}: {model.step} steps in {wall:.1f} s, "
+          f"{len(out)} tips", file=sys.stderr, flush=True)
+    return dict(act=np.asarray(act.output, dtype=np.float64), tips=_tip_track(out),
+                n_tips=int(len(out)), periods=periods, wall=wall, steps=int(model.step),
+                finite=bool(np.all(np.isfinite(model.u))))
+
+
+def c2_spiral_leg(s2_times=(310.0, 340.0), t_max: float = 1000.0):
+    """C2 at its full length: TP06 1026^2, 30 deg fibres, S1 planar on the
+    left 10 columns at t = 0, S2 on the lower-left quadrant, 100,000 steps,
+    run by model.run() in FP64 and in FP32 (FP64 calcium) with three
+    trackers: activation time (threshold -40 mV), spiral tips
+    (SpiralWaveCore2DTracker, every 1 ms from S2 + 10 ms) and activation
+    periods at five detectors (Period2DTracker, every 0.1 ms).  S2 at 310 ms
+    as configured, and at 340 ms: at 310 ms the S1 wave has not yet crossed
+    the S2 quadrant's edge and the S2 wave dies out (no tip is ever found);
+    340 ms is the earliest S2 of 310, 340, 370, 400 and 430 ms that leaves a
+    rotating wave.  Spiral dynamics diverge chaotically, so the precisions
+    are compared by statistics: the S1 front's activation times (nodes
+    activated before S2) within one dt, the tip trajectories (distance of
+    the FP64 and FP32 tip centroids over time, and the time they first part
+    by 5 nodes) and the median activation period at the detectors."""
+    n = N_SIDE + 2
+    cells = [[n // 4, n // 4], [n // 2, n // 2], [3 * n // 4, n // 4], [n // 4, 3 * n // 4],
+             [3 * n // 4, 3 * n // 4]]
+    dt = 0.01
+    legs = {}
+    for s2 in s2_times:
+        runs = {prec: _c2_spiral_run(prec, s2, t_max, cells) for prec in ("fp64", "fp32")}
+        a, b = runs["fp64"]["act"], runs["fp32"]["act"]
+        s1 = (a >= 0) & (a < s2)
+        both = s1 & (b >= 0)
+        dlat = np.abs(a[both] - b[both]) if both.any() else np.zeros(1)
+        t64, t32 = runs["fp64"]["tips"], runs["fp32"]["tips"]
+        common = sorted(set(t64) & set(t32))
+        dist = np.array([math.hypot(t64[t][0] - t32[t][0], t64[t][1] - t32[t][1])
+                         for t in common])
+        apart = [t for t, d in zip(common, dist) if d > 5.0]
+
+        def med_period(r):
+            vals = [float(np.median(p[p > 0])) for p in r["periods"] if np.any(p > 0)]
+            return round(float(np.median(vals)), 2) if vals else None
+
+        legs[f"s2_{int(s2)}ms"] = dict(
+            steps={k: r["steps"] for k, r in runs.items()},
+            wall_s={k: round(r["wall"], 2) for k, r in runs.items()},
+            ms_per_step={k: round(r["wall"] / max(1, r["steps"]) * 1e3, 4)
+                         for k, r in runs.items()},
+            finite={k: r["finite"] for k, r in runs.items()},
+            s1_front=dict(nodes=int(s1.sum()), both_activated=int(both.sum()),
+                          missing_in_fp32=int((s1 & (b < 0)).sum()),
+                          lat_max_dt=round(float(dlat.max()) / dt, 3),
+                          beyond_1dt=int(np.count_nonzero(dlat > dt + 1e-9))),
+            tips=dict(found={k: r["n_tips"] for k, r in runs.items()},
+                      tracked_times_with_tips={k: len(r["tips"]) for k, r in runs.items()},
+                      common_times=len(common),
+                      centroid_dist_nodes_p50_p90=[round(float(x), 2) for x in
+                                                   np.percentile(dist, [50, 90])]
+                      if len(dist) else None,
+                      first_apart_5_nodes_ms=apart[0] if apart else None,
+                      last_common_ms=common[-1] if common else None),
+            median_period_ms={k: med_period(r) for k, r in runs.items()})
+    return dict(workload="TP06 2D 1024x1024 (+boundary), 9-point, fibre 30 deg, S1 planar t=0, "
+                         "S2 lower-left quadrant, 100,000 steps, model.run() with activation, "
+                         "spiral-tip and period trackers, FP64 and FP32",
+                **legs)
 
 
 def c3_leg(steps: int = 400, warmup: int = 40):
@@ -368,7 +476,10 @@ def ensemble_leg(n_tissues: int, steps: int, full: bool):
         b.admit(s)
     torch.cuda.synchronize()
     t_adm = time.perf_counter() - t_adm
-    b.advance(200)  # S1 at step 0, graph capture, warm-up
+    # S1 at step 0; the autotuner times every launch shape on the batch's
+    # first steps; then the graphs of the chosen shape are captured: all
+    # before the timed window
+    b.advance((b.tune_steps + 1) * len(b.variants()) + 200)
     torch.cuda.synchronize()
     # events on the batch's own stream, the one its kernels are launched on
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -402,6 +513,7 @@ def ensemble_leg(n_tissues: int, steps: int, full: bool):
                          "bytes_per_step": bytes_step},
                projected_s=round(total_cell_steps / (n_active / sec), 1),
                launch_variant=b.variant,
+               steps_per_launch=(2 << ((b.variant >> 4) & 3)) if b.variant & 8 else 1,
                tuning_ms={str(k): round(v, 4) for k, v in (b.tuning or {}).items()})
     if full:
         w0 = time.perf_counter()
@@ -574,7 +686,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-fp64-serving", action="store_true")
     ap.add_argument("--per-request", action="store_true",
                     help="list each served request's errors against the FP64 path")
-    ap.add_argument("--legs", default="stencil,ensemble,serving,c1,c3")
+    ap.add_argument("--legs", default="stencil,ensemble,serving,c1,c2spiral,c3")
     args = ap.parse_args(argv)
     legs = set(args.legs.split(","))
     if args.gpus != 1:
@@ -657,6 +769,9 @@ def main(argv=None) -> int:
     if "c1" in legs:
         line["c1"] = c1_leg()
         print(f"# c1: {line['c1']}", file=sys.stderr, flush=True)
+    if "c2spiral" in legs:
+        line["c2_spiral"] = c2_spiral_leg()
+        print(f"# c2 spiral: {line['c2_spiral']}", file=sys.stderr, flush=True)
     if "c3" in legs:
         line["c3"] = c3_leg()
         print(f"# c3: {line['c3']}", file=sys.stderr, flush=True)

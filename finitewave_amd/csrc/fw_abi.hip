@@ -77,6 +77,11 @@ static void derive_tp06(Params& P) {
 
 void complete_params(int model, Params& P) {
   if (model == 6) derive_tp06(P);
+  if (model == 2) {
+    // Mitchell-Schaeffer, float path: 1 / tau_open, 1 / tau_close
+    P.v[5] = 1.0 / P.v[1];
+    P.v[6] = 1.0 / P.v[0];
+  }
   for (int q = 0; q < 64; ++q) P.f[q] = (float)P.v[q];  // round to nearest
 }
 

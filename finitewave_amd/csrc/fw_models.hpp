@@ -221,7 +221,15 @@ __device__ __forceinline__ R ionic_ms(R u, R un, SP& S, int64_t c,
           tau_out = par<R>(P, 3), u_gate = par<R>(P, 4);
   auto&& href = ref<R>(S, 0, c);
   R h = href;
-  h += (u < u_gate) ? dt * (R(1) - h) / tau_open : dt * (-h) / tau_close;
+  if constexpr (sizeof(R) == 4) {
+    // float path: multiplications by 1 / tau_open and 1 / tau_close, rounded
+    // once on the host (parameters 5 and 6), instead of a reciprocal per
+    // node of the tau the select picks
+    const R inv_open = par<R>(P, 5), inv_close = par<R>(P, 6);
+    h += (u < u_gate) ? dt * (R(1) - h) * inv_open : dt * (-h) * inv_close;
+  } else {
+    h += (u < u_gate) ? dt * (R(1) - h) / tau_open : dt * (-h) / tau_close;
+  }
   href = h;
   R C = (u * u) * (R(1) - u);  // (u**2)*(1-u), u**2 -> u*u
   R J_in = h * C / tau_in;
