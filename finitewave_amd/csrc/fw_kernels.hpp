@@ -178,6 +178,7 @@ struct KArgs {
   int32_t n_stim;
   int32_t stim_end;
   const int32_t* stim_end_in;
+  int32_t stim_lo[3], stim_hi[3];  // the union of the entries' boxes
   StimRec stim[kMaxStims];
 };
 
@@ -913,6 +914,14 @@ __device__ __forceinline__ bool stim_due(const A& a, int32_t s) {
 // a conducting node (i, j, k) = c holds before step s: the operations of
 // Stim*.stimulate on that node (u = V; u += dt*I, then u_max), in the
 // potential's storage type as fw_stim_apply does them
+// Whether node (i, j, k) lies in the union of the scheduled stimuli's boxes:
+// most nodes fail this test and skip the per-entry loop
+template <typename A>
+__device__ __forceinline__ bool in_stim_box(const A& a, int64_t i, int64_t j, int64_t k) {
+  return i >= a.stim_lo[0] && i < a.stim_hi[0] && j >= a.stim_lo[1] && j < a.stim_hi[1] &&
+         k >= a.stim_lo[2] && k < a.stim_hi[2];
+}
+
 template <typename U, typename A>
 __device__ __forceinline__ U stim_node(const A& a, int32_t s, int64_t i, int64_t j, int64_t k,
                                        int64_t c, U v) {
@@ -973,7 +982,7 @@ __device__ __forceinline__ void step_node(const KArgs<R, UType<MODEL, R>>& a, in
       if (stim_due(a, step + 1)) {
         int64_t si, sj, sk;
         node_coords(a, c, si, sj, sk);
-        v = stim_node<U>(a, step + 1, si, sj, sk, c, v);
+        if (in_stim_box(a, si, sj, sk)) v = stim_node<U>(a, step + 1, si, sj, sk, c, v);
       }
       a.u_new[c] = v;
     }
@@ -982,7 +991,7 @@ __device__ __forceinline__ void step_node(const KArgs<R, UType<MODEL, R>>& a, in
     // to the value its u_new buffer holds (the next step's potential)
     int64_t i, j, k;
     node_coords(a, c, i, j, k);
-    a.u_new[c] = stim_node<U>(a, step + 1, i, j, k, c, a.u_new[c]);
+    if (in_stim_box(a, i, j, k)) a.u_new[c] = stim_node<U>(a, step + 1, i, j, k, c, a.u_new[c]);
   }
   probe_node<R, MODEL, PREC>(a, c, code, (double)uc);
 }

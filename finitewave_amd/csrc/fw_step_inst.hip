@@ -156,12 +156,18 @@ static void fill_args(const fw_plan* plan, const void* u, void* u_new, const Sta
 // the plan's stimulus schedule into the step kernel's arguments
 static void fill_stims(const fw_plan* plan, KArgs<Real, UReal>& a) {
   a.n_stim = plan->n_stims;
+  for (int x = 0; x < 3; ++x) {
+    a.stim_lo[x] = INT32_MAX;
+    a.stim_hi[x] = INT32_MIN;
+  }
   for (int q = 0; q < plan->n_stims; ++q) {
     const fw_stim_sched& ss = plan->stims[q];
     StimRec& r = a.stim[q];
     for (int x = 0; x < 3; ++x) {
       r.lo[x] = (int32_t)ss.stim.lo[x];
       r.hi[x] = (int32_t)ss.stim.hi[x];
+      a.stim_lo[x] = r.lo[x] < a.stim_lo[x] ? r.lo[x] : a.stim_lo[x];
+      a.stim_hi[x] = r.hi[x] > a.stim_hi[x] ? r.hi[x] : a.stim_hi[x];
     }
     r.kind = ss.stim.kind;
     r.clamp = ss.stim.clamp;
