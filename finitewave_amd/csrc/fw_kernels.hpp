@@ -20,6 +20,8 @@
 // hit that XCD's own L2).
 #pragma once
 
+#include <utility>
+
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -181,6 +183,44 @@ struct KArgs {
   int32_t stim_lo[3], stim_hi[3];  // the union of the entries' boxes
   StimRec stim[kMaxStims];
 };
+
+// Lane shifts across the whole 64-lane wavefront by DPP (v_mov_b32_dpp
+// wave_shr:1 / wave_shl:1): a VALU move, no LDS round trip.
+// lane i receives lane i-1's value (lane 0 receives 0)
+template <typename T>
+__device__ __forceinline__ T from_west(T x) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int32_t, x), 0x138, 0xf, 0xf, false));
+  } else {
+    const int64_t b = __builtin_bit_cast(int64_t, x);
+    const int32_t lo = __builtin_amdgcn_update_dpp(0, (int32_t)b, 0x138, 0xf, 0xf, false);
+    const int32_t hi = __builtin_amdgcn_update_dpp(0, (int32_t)(b >> 32), 0x138, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, ((int64_t)hi << 32) | (int64_t)(uint32_t)lo);
+  }
+}
+
+// lane i receives lane i+1's value (lane 63 receives 0)
+template <typename T>
+__device__ __forceinline__ T from_east(T x) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int32_t, x), 0x130, 0xf, 0xf, false));
+  } else {
+    const int64_t b = __builtin_bit_cast(int64_t, x);
+    const int32_t lo = __builtin_amdgcn_update_dpp(0, (int32_t)b, 0x130, 0xf, 0xf, false);
+    const int32_t hi = __builtin_amdgcn_update_dpp(0, (int32_t)(b >> 32), 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(T, ((int64_t)hi << 32) | (int64_t)(uint32_t)lo);
+  }
+}
+
+// Runs f(integral_constant<0>) .. f(integral_constant<P-1>) while each
+// returns true: the body of a loop unrolled P times whose phase is known at
+// compile time.
+template <typename F, int... I>
+__device__ __forceinline__ bool tb_phases(F&& f, std::integer_sequence<int, I...>) {
+  return (f(std::integral_constant<int, I>{}) && ...);
+}
 
 // (i, j, k) of flat node index c in a C-order grid (k = 0 in 2D)
 template <typename A>

@@ -22,7 +22,8 @@
 // loads plane i+1 at rows j-1, j, j+1 and columns k, k+1, computes five
 // squares and the fluxes across its upper x, y and z faces and across the
 // upper y face of the node below it in j; the lower-x quantities come from
-// the previous plane (registers), the lower-z ones from lane - 1 (a shuffle).
+// the previous plane (registers), the lower-z ones from lane - 1 (a DPP
+// wavefront shift).
 // The face fluxes are the reference operator's; their rounding differs from
 // the per-node kernel's (k_step) by the order of the sums.
 #pragma once
@@ -63,19 +64,25 @@ __device__ __forceinline__ float qmean(Sq q) {
   return q.s * (q.m > 3.5f ? 0.25f : (1.0f / 3.0f));
 }
 
+// a square as the cross terms read it: its mean and whether at least three
+// of its nodes conduct, formed once per square (each square serves two faces)
+struct Qv {
+  float q;
+  float ok;  // 1 or 0
+};
+
+__device__ __forceinline__ Qv qv(Sq s) { return Qv{qmean(s), s.m > 2.5f ? 1.0f : 0.0f}; }
+
 // a cross term of the face (c, c+a) between its two squares; mf = m(c) m(c+a)
-__device__ __forceinline__ float cross_term(float d, float mf, Sq hi, Sq lo) {
+__device__ __forceinline__ float cross_term(float d, float mf, Qv hi, Qv lo) {
 #pragma clang fp contract(fast)
-  const bool on = mf > 0.5f && hi.m > 2.5f && lo.m > 2.5f;
-  return on ? d * (qmean(hi) - qmean(lo)) : 0.0f;
+  const bool on = mf * hi.ok * lo.ok > 0.5f;
+  return on ? d * (hi.q - lo.q) : 0.0f;
 }
 
-__device__ __forceinline__ Sq shfl_up1(Sq q) {
-  Sq r;
-  r.s = __shfl_up(q.s, 1, 64);
-  r.m = __shfl_up(q.m, 1, 64);
-  return r;
-}
+// lane - 1's square by a DPP wavefront shift (lane 0 receives zeros; its
+// results are not kept)
+__device__ __forceinline__ Qv shfl_up1(Qv a) { return Qv{from_west(a.q), from_west(a.ok)}; }
 
 template <int MODEL, int FL>
 __global__ __launch_bounds__(kBlock) void k_step3_march(const KArgs<float, float> a,
@@ -128,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void k_step3_march(const KArgs<float, float
   float m2p = (float)(code[c + op + ok] & 1);
   // the lower-x quantities: those of plane i - 1
   float fx_lo = 0.0f;
-  Sq xy_lo{0.0f, 0.0f}, xym_lo{0.0f, 0.0f}, xz_lo{0.0f, 0.0f};
+  Qv xy_lo{0.0f, 0.0f}, xym_lo{0.0f, 0.0f}, xz_lo{0.0f, 0.0f};
   for (int i = ibeg; i < i1; ++i, c += si) {
     // plane i + 1 (none after the last plane)
     float n0 = u1, n1 = u1, n2 = u1, n0p = u1, n1p = u1, n2p = u1;
@@ -144,13 +151,13 @@ __global__ __launch_bounds__(kBlock) void k_step3_march(const KArgs<float, float
       q2p = (float)(code[cn + op + ok] & 1);
     }
     // the squares this node starts: S_xy(c), S_xy(c-y), S_xz(c), S_yz(c), S_yz(c-y)
-    const Sq xy = square4(m1, u1, q1, n1, m2, u2, q2, n2);
-    const Sq xym = square4(m0, u0, q0, n0, m1, u1, q1, n1);
-    const Sq xz = square4(m1, u1, q1, n1, m1p, u1p, q1p, n1p);
-    const Sq yz = square4(m1, u1, m2, u2, m1p, u1p, m2p, u2p);
-    const Sq yzm = square4(m0, u0, m1, u1, m0p, u0p, m1p, u1p);
+    const Qv xy = qv(square4(m1, u1, q1, n1, m2, u2, q2, n2));
+    const Qv xym = qv(square4(m0, u0, q0, n0, m1, u1, q1, n1));
+    const Qv xz = qv(square4(m1, u1, q1, n1, m1p, u1p, q1p, n1p));
+    const Qv yz = qv(square4(m1, u1, m2, u2, m1p, u1p, m2p, u2p));
+    const Qv yzm = qv(square4(m0, u0, m1, u1, m0p, u0p, m1p, u1p));
     // those of the node at k - 1 (lane - 1)
-    const Sq xz_z = shfl_up1(xz), yz_z = shfl_up1(yz), yzm_z = shfl_up1(yzm);
+    const Qv xz_z = shfl_up1(xz), yz_z = shfl_up1(yz), yzm_z = shfl_up1(yzm);
     // fluxes across the upper x, y, z faces of c and the upper y face of c - y
     const float mx = m1 * q1, my = m1 * m2, mz = m1 * m1p, my0 = m0 * m1;
     const float fx = dxx * mx * (n1 - u1) + cross_term(dxy, mx, xy, xym) +
@@ -161,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_step3_march(const KArgs<float, float
                       cross_term(dyz, my0, yzm, yzm_z);
     const float fz = dzz * mz * (u1p - u1) + cross_term(dzx, mz, xz, xz_lo) +
                      cross_term(dzy, mz, yz, yzm);
-    const float fz0 = __shfl_up(fz, 1, 64);
+    const float fz0 = from_west(fz);
     if (own && i >= i0) {
       const float uc = u1;
       if (a.act_on) {
