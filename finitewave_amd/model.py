@@ -30,17 +30,9 @@ def _gate_open(tracker, t, step) -> bool:
 
 
 def _saver_due(saver, model) -> bool:
-    savers = getattr(saver, "savers", None)
-    if savers is not None:
-        return any(_saver_due(s, model) for s in savers)
-    # state_saver.py:52-77
-    if saver.passed:
-        return False
-    if saver.time < 0 and model.t < model.t_max:
-        return False
-    if saver.time >= 0 and model.t < saver.time:
-        return False
-    return True
+    """Whether the state saver writes at the model's current time: the
+    saver's own gate (StateSaver.due)."""
+    return saver.due(model)
 
 
 class CardiacModel:

@@ -22,12 +22,17 @@ class StateSaver:
         self.model = model
         self.passed = self.path == ""
 
-    def save(self):
+    def due(self, model) -> bool:
+        """Whether save() would write at `model`'s current time (the gate of
+        state_saver.py:52-77): once, at t_max or from `time` on."""
         if self.passed:
-            return
-        if self.time < 0 and self.model.t < self.model.t_max:
-            return
-        if self.time >= 0 and self.model.t < self.time:
+            return False
+        if self.time < 0 and model.t < model.t_max:
+            return False
+        return not (self.time >= 0 and model.t < self.time)
+
+    def save(self):
+        if not self.due(self.model):
             return
         if not Path(self.path).exists():
             Path(self.path).mkdir(parents=True, exist_ok=True)
@@ -50,6 +55,9 @@ class StateSaverCollection(StateSaver):
     def initialize(self, model):
         for saver in self.savers:
             saver.initialize(model)
+
+    def due(self, model) -> bool:
+        return any(saver.due(model) for saver in self.savers)
 
     def save(self):
         for saver in self.savers:
