@@ -55,12 +55,14 @@ DR = 0.25  # 0.025 cm in the model's millimetres
 # algorithmic bytes per active node and step: the state the reference's step
 # reads + the state it writes (TP06 reads u and 18 state arrays, writes u_new,
 # 17 of them and oo; cai is read-only).  FP64: 38 arrays x 8 B.  Mixed FP32
-# form: FP32 state with cai, casr, cass, nai, Ki in FP64 = 188 B.  Plus the
-# 1-byte node code per node and the 4-byte activation word per active node.
-B_STATE = {"fp64": 304, "fp32": 188}
-# bytes the FP32 kernel stores per node: it also keeps u, h and j in FP64
-# (a float potential and float slow sodium gates stop moving near rest)
-B_STORED = {"fp64": 304, "fp32": 212}
+# form: FP32 state with the calcium concentrations cai, casr, cass in FP64 =
+# 172 B.  Plus the 1-byte node code per node and the 4-byte activation word
+# per active node.
+B_STATE = {"fp64": 304, "fp32": 172}
+# bytes the FP32 kernel stores per node: it also keeps u in FP64 (a float
+# potential stops following the resting state); nai, Ki, h and j are floats
+# holding differences from their initial values
+B_STORED = {"fp64": 304, "fp32": 180}
 B_TRACK = 4
 
 

@@ -24,11 +24,12 @@ CODE_CONDUCT, CODE_FROZEN, CODE_PROBE = 1, 2, 4
 MAX_STATE, MAX_PARAMS, MAX_FIELDS, MAX_STIMS = 24, 64, 9, 8
 NOALLOC_SCRATCH_BYTES = 4 << 20
 BATCH_ALT_STATE = 4
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 # every symbol include/finitewave_amd.h declares
 EXPORTS = (
-    "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64", "fw_u_is_f64",
+    "fw_abi_version", "fw_last_error", "fw_model_info", "fw_state_is_f64", "fw_state_offset",
+    "fw_u_is_f64",
     "fw_compute_weights", "fw_diffusion", "fw_ionic", "fw_plan_create",
     "fw_plan_destroy", "fw_plan_set_weights_k", "fw_run", "fw_run_graph",
     "fw_run_graph_prepare", "fw_run_noalloc",
@@ -100,6 +101,7 @@ def lib():
     L.fw_last_error.restype = ctypes.c_char_p
     L.fw_model_info.argtypes = [i32, POINTER(i32), POINTER(i32)]
     L.fw_state_is_f64.argtypes = [i32, i32, i32]
+    L.fw_state_offset.argtypes = [i32, i32, i32, POINTER(dbl)]
     L.fw_u_is_f64.argtypes = [i32, i32]
     L.fw_compute_weights.argtypes = [i32, i32, vp, vp, POINTER(vp), POINTER(i64), vp]
     L.fw_diffusion.argtypes = [i32, i32, vp, vp, vp, vp, i64, POINTER(i64), vp]
@@ -150,6 +152,13 @@ def state_is_f64(model_id: int, precision: int, k: int) -> bool:
     if rc < 0:
         check(rc, "fw_state_is_f64")
     return bool(rc)
+
+
+def state_offset(model_id: int, precision: int, k: int) -> float:
+    """The value state k is stored relative to (the array holds value - offset)."""
+    out = c_double()
+    check(lib().fw_state_offset(model_id, precision, k, ctypes.byref(out)), "fw_state_offset")
+    return out.value
 
 
 def u_is_f64(model_id: int, precision: int) -> bool:

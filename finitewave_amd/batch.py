@@ -233,6 +233,8 @@ class TissueBatch:
         self.names = self.proto._state_call_order()
         self.f64 = [_native.state_is_f64(self.model_id, self.prec, k)
                     for k in range(len(self.names))]
+        self.off = [_native.state_offset(self.model_id, self.prec, k)
+                    for k in range(len(self.names))]
         # the multi-step launch shapes (light models) write a second state set
         self.alt_state = self.model_id <= 4
         self._flags = None   # pinned host flags of the divergence check
@@ -278,9 +280,9 @@ class TissueBatch:
         u = torch.full(shape, float(p.init_u), dtype=rd, device=self.dev)
         un = u.clone() if p._copy_u_new else torch.zeros(shape, dtype=rd, device=self.dev)
         inits = {name: getattr(p, init_name) for name, init_name, _ in p._state}
-        state = [torch.full(shape, float(inits[nm]), device=self.dev,
+        state = [torch.full(shape, float(inits[nm]) - off, device=self.dev,
                             dtype=torch.float64 if f else torch.float32)
-                 for nm, f in zip(self.names, self.f64)]
+                 for nm, f, off in zip(self.names, self.f64, self.off)]
         state_alt = [torch.empty_like(x) for x in state] if self.alt_state else []
         code = torch.from_numpy(tissue.node_codes()).to(self.dev)
         n_steps = run_length(sim.t_max, self.dt, self._t_fold(int(math.ceil(sim.t_max / self.dt)) + 1))

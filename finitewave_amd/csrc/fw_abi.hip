@@ -248,6 +248,16 @@ int fw_state_is_f64(int32_t model, int32_t precision, int32_t k) {
   return state_is_f64(model, precision, k) ? 1 : 0;
 }
 
+int fw_state_offset(int32_t model, int32_t precision, int32_t k, double* offset) {
+  if (model < 0 || model >= FW_N_MODELS || k < 0 || k >= model_n_state(model) ||
+      (precision != FW_PREC_FP64 && precision != FW_PREC_FP32) || !offset) {
+    set_error("fw_state_offset: bad arguments");
+    return FW_E_INVALID;
+  }
+  *offset = state_offset(model, precision, k);
+  return FW_OK;
+}
+
 int fw_compute_weights(int32_t ndim, int32_t kind, double* w, const int8_t* m,
                        const double* const* d, const int64_t* shape, void* stream) {
   if (!w || !m || !d || !shape || (ndim != 2 && ndim != 3)) {

@@ -523,7 +523,7 @@ template <typename R, int MODEL>
 __device__ __forceinline__ double state_as_double(const StatePtrs& S, int k,
                                                   int64_t c, int prec) {
   if (state_is_f64(MODEL, prec, k)) return static_cast<const double*>(S.p[k])[c];
-  return (double)static_cast<const float*>(S.p[k])[c];
+  return (double)static_cast<const float*>(S.p[k])[c] + state_offset(MODEL, prec, k);
 }
 
 // The 3x3 neighbourhood of an interior 2D node: potentials and conduct bits

@@ -88,6 +88,29 @@ __device__ __forceinline__ R divc(R a, R b) {
   else return a / b;
 }
 
+// a / b for a node-dependent b.  FP64: the core of the IEEE division
+// sequence, without its operand scaling (two v_div_scale), its scaled
+// correction and its special-case fix-up: r = 1/b from the hardware estimate
+// refined by two Newton steps, q = a r, then q + (a - b q) r.  These are the
+// operations and roundings of the full sequence, so the quotient is the same
+// bits wherever the full sequence would not rescale: a, b, 1/b and a/b normal
+// and their exponents less than 768 apart (the models' rates, gates,
+// concentrations and time constants stay far inside that while the state is
+// finite).  It differs for b infinite (NaN instead of 0) and for a = -0
+// (+0).  FP32: the plain division.
+template <typename R>
+__device__ __forceinline__ R dv(R a, R b) {
+  if constexpr (sizeof(R) == 8) {
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    r = fma(fma(-b, r, 1.0), r, r);
+    const double q = a * r;
+    return fma(fma(-b, q, a), r, q);
+  } else {
+    return a / b;
+  }
+}
+
 // TP06 quantities that are the same for every node (functions of the
 // parameters only), appended after its 49 parameters by complete_params on the
 // host, in the operation order of the expressions they replace: IEEE +, *, /
@@ -168,21 +191,37 @@ __device__ __forceinline__ T& ref(RegState<R, NS>& S, int k, int64_t) {
   return S.v[k];
 }
 
+// Stored form of state k: its value minus state_offset(model, prec, k).  The
+// FP32 path keeps TP06's nai, Ki (k = 3, 4) and sodium inactivation gates h,
+// j (k = 6, 7) as float differences from their initial values
+// (tp06_2d.py:155-159); every other state, and every state of the FP64 path,
+// is stored as its value.  A float near 138.3 resolves 1.5e-5 and one near
+// 0.75 resolves 6e-8; the differences stay small over a simulation (the
+// concentrations drift slowly, the gates rest near 0.72-0.77), where their
+// ulp is a few hundred to tens of thousands of times finer.
+__host__ __device__ constexpr double state_offset(int model, int prec, int k) {
+  return (prec == 1 && model == 6)
+             ? (k == 3 ? 7.67 : k == 4 ? 138.3 : (k == 6 || k == 7) ? 0.75 : 0.0)
+             : 0.0;
+}
+
 // Rush-Larsen gate update g <- g_inf - (g_inf - g) * exp(-dt / tau) (e.g.
-// tp06_2d.py:307-309).  FP64 path: that expression.  A gate stored double on
-// the FP32 path is advanced by its increment, g + (g_inf - g) * (1 - e) with
-// 1 - e = -expm1(-dt / tau) in float and the sum in double: a float gate
-// near its steady value stops moving once the increment falls below half an
-// ulp, and 1 - e formed from a float e loses its digits for slow gates; both
-// biases shift the resting sodium availability (TP06 j by 2.7e-4 after
-// 780 ms) and with it the conduction velocity.
-template <typename R, typename G>
-__device__ __forceinline__ G rl_gate(G g, R inf, R tau, R dt) {
-  if constexpr (sizeof(R) == 4 && sizeof(G) == 8) {
+// tp06_2d.py:307-309) of a gate stored as d = g - g0.  FP64 path (g0 = 0):
+// that expression.  FP32 path: d advances by the increment,
+// d + ((g_inf - g0) - d) * (1 - e) with 1 - e = -expm1(-dt / tau).  A float
+// gate near its steady value stops moving once the increment falls below half
+// an ulp of the gate: TP06's j at rest, dt / tau_j = 1.4e-4, stalled anywhere
+// within 2.1e-4 of j_inf, its availability drifted by 2.7e-4 after 780 ms and
+// the conduction velocity with it.  Stored as d the stall band is half an ulp
+// of d over dt / tau (6.5e-6 for |d| < 0.031), and 1 - e taken from expm1
+// keeps the digits a float e would lose for slow gates.
+template <typename R>
+__device__ __forceinline__ R rl_gate(R d, R inf, R tau, R dt, R g0) {
+  if constexpr (sizeof(R) == 4) {
     const R em = -expm1f(-dt / tau);
-    return g + ((G)inf - g) * (G)em;
+    return d + ((inf - g0) - d) * em;
   } else {
-    return inf - (inf - g) * fexp(-dt / tau);
+    return inf - (inf - d) * fexp(dv(-dt, tau));
   }
 }
 
@@ -465,65 +504,71 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     else return x / (R_ * T);
   };
 
+  // nai, Ki: stored as differences from state_offset on the FP32 path
+  typedef std::conditional_t<F64, double, float> Sc;
+  constexpr D kNai0 = state_offset(6, F64 ? 0 : 1, 3), kKi0 = state_offset(6, F64 ? 0 : 1, 4);
   auto&& cair = ref<D>(S, 0, c);
   auto&& casrr = ref<D>(S, 1, c);
   auto&& cassr = ref<D>(S, 2, c);
-  auto&& nair = ref<D>(S, 3, c);
-  auto&& Kir = ref<D>(S, 4, c);
-  const D caiD = cair, casrD = casrr, cassD = cassr, naiD = nair, KiD = Kir;
+  auto&& nair = ref<Sc>(S, 3, c);
+  auto&& Kir = ref<Sc>(S, 4, c);
+  const D caiD = cair, casrD = casrr, cassD = cassr;
+  const D naiS = (D)(Sc)nair, KiS = (D)(Sc)Kir;
+  const D naiD = F64 ? naiS : naiS + kNai0, KiD = F64 ? KiS : KiS + kKi0;
   const R cai = (R)caiD, cass = (R)cassD, nai = (R)naiD, Ki = (R)KiD;
 
-  const R Ek = RTONF * flog(ko / Ki);
-  const R Ena = RTONF * flog(nao / nai);
-  const R Eks = RTONF * flog((ko + pKNa * nao) / (Ki + pKNa * nai));
-  const R Eca = R(0.5) * RTONF * flog(cao / cai);
+  const R Ek = RTONF * flog(dv(ko, Ki));
+  const R Ena = RTONF * flog(dv(nao, nai));
+  const R Eks = RTONF * flog(dv(ko + pKNa * nao, Ki + pKNa * nai));
+  const R Eca = R(0.5) * RTONF * flog(dv(cao, cai));
 
   // calc_ina
   R ina;
   {
-    // h, j: stored double on the FP32 path as well (state_is_f64)
-    typedef std::conditional_t<sizeof(R) == 4, double, R> G;
+    // h, j: stored as differences from state_offset on the FP32 path
+    constexpr R kH0 = (R)state_offset(6, F64 ? 0 : 1, 6), kJ0 = (R)state_offset(6, F64 ? 0 : 1, 7);
     auto&& mr = ref<R>(S, 5, c);
-    auto&& hr = ref<G>(S, 6, c);
-    auto&& jr = ref<G>(S, 7, c);
-    const R alpha_m = R(1.0) / (R(1.0) + fexp(divc(R(-60.0) - u, R(5.0))));
-    const R beta_m = R(0.1) / (R(1.0) + fexp(divc(u + R(35.0), R(5.0)))) +
-                     R(0.1) / (R(1.0) + fexp(divc(u - R(50.0), R(200.0))));
+    auto&& hr = ref<R>(S, 6, c);
+    auto&& jr = ref<R>(S, 7, c);
+    const R alpha_m = dv(R(1.0), R(1.0) + fexp(divc(R(-60.0) - u, R(5.0))));
+    const R beta_m = dv(R(0.1), R(1.0) + fexp(divc(u + R(35.0), R(5.0)))) +
+                     dv(R(0.1), R(1.0) + fexp(divc(u - R(50.0), R(200.0))));
     const R tau_m = alpha_m * beta_m;
-    const R m_inf = R(1.0) / ((R(1.0) + fexp(divc(R(-56.86) - u, R(9.03)))) *
+    const R m_inf = dv(R(1.0), (R(1.0) + fexp(divc(R(-56.86) - u, R(9.03)))) *
                               (R(1.0) + fexp(divc(R(-56.86) - u, R(9.03)))));
     R alpha_h, beta_h;
     if (u >= R(-40.0)) {
       alpha_h = R(0.0);
-      beta_h = R(0.77) / (R(0.13) * (R(1.0) + fexp(divc(-(u + R(10.66)), R(11.1)))));
+      beta_h = dv(R(0.77), R(0.13) * (R(1.0) + fexp(divc(-(u + R(10.66)), R(11.1)))));
     } else {
       alpha_h = R(0.057) * fexp(divc(-(u + R(80.0)), R(6.8)));
       beta_h = R(2.7) * fexp(R(0.079) * u) + R(310000.0) * fexp(R(0.3485) * u);
     }
-    const R tau_h = R(1.0) / (alpha_h + beta_h);
-    const R h_inf = R(1.0) / ((R(1.0) + fexp(divc(u + R(71.55), R(7.43)))) *
+    const R tau_h = dv(R(1.0), alpha_h + beta_h);
+    const R h_inf = dv(R(1.0), (R(1.0) + fexp(divc(u + R(71.55), R(7.43)))) *
                               (R(1.0) + fexp(divc(u + R(71.55), R(7.43)))));
     R alpha_j, beta_j;
     if (u >= R(-40.0)) {
       alpha_j = R(0.0);
-      beta_j = R(0.6) * fexp(R(0.057) * u) /
-               (R(1.0) + fexp(R(-0.1) * (u + R(32.0))));
+      beta_j = dv(R(0.6) * fexp(R(0.057) * u),
+                R(1.0) + fexp(R(-0.1) * (u + R(32.0))));
     } else {
-      alpha_j = (R(-25428.0) * fexp(R(0.2444) * u) -
-                 R(6.948e-06) * fexp(R(-0.04391) * u)) *
-                (u + R(37.78)) / (R(1.0) + fexp(R(0.311) * (u + R(79.23))));
-      beta_j = R(0.02424) * fexp(R(-0.01052) * u) /
-               (R(1.0) + fexp(R(-0.1378) * (u + R(40.14))));
+      alpha_j = dv((R(-25428.0) * fexp(R(0.2444) * u) -
+                    R(6.948e-06) * fexp(R(-0.04391) * u)) *
+                       (u + R(37.78)),
+                   R(1.0) + fexp(R(0.311) * (u + R(79.23))));
+      beta_j = dv(R(0.02424) * fexp(R(-0.01052) * u),
+                R(1.0) + fexp(R(-0.1378) * (u + R(40.14))));
     }
-    const R tau_j = R(1.0) / (alpha_j + beta_j);
+    const R tau_j = dv(R(1.0), alpha_j + beta_j);
     const R j_inf = h_inf;
-    const R m = m_inf - (m_inf - mr) * fexp(-dt / tau_m);
-    const G hg = rl_gate<R, G>(hr, h_inf, tau_h, dt);
-    const G jg = rl_gate<R, G>(jr, j_inf, tau_j, dt);
+    const R m = m_inf - (m_inf - mr) * fexp(dv(-dt, tau_m));
+    const R hd = rl_gate<R>(hr, h_inf, tau_h, dt, kH0);
+    const R jd = rl_gate<R>(jr, j_inf, tau_j, dt, kJ0);
     mr = m;
-    hr = hg;
-    jr = jg;
-    const R h = (R)hg, j = (R)jg;
+    hr = hd;
+    jr = jd;
+    const R h = F64 ? hd : hd + kH0, j = F64 ? jd : jd + kJ0;
     ina = gna * m * m * m * h * j * (u - Ena);
   }
   // calc_ical
@@ -533,27 +578,27 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     auto&& fr = ref<R>(S, 14, c);
     auto&& f2r = ref<R>(S, 15, c);
     auto&& fcr = ref<R>(S, 16, c);
-    const R d_inf = R(1.0) / (R(1.0) + fexp(divc(R(-8) - u, R(7.5))));
-    const R Ad = R(1.4) / (R(1.0) + fexp(divc(R(-35) - u, R(13)))) + R(0.25);
-    const R Bd = R(1.4) / (R(1.0) + fexp(divc(u + R(5), R(5))));
-    const R Cd = R(1.0) / (R(1.0) + fexp(divc(R(50) - u, R(20))));
+    const R d_inf = dv(R(1.0), R(1.0) + fexp(divc(R(-8) - u, R(7.5))));
+    const R Ad = dv(R(1.4), R(1.0) + fexp(divc(R(-35) - u, R(13)))) + R(0.25);
+    const R Bd = dv(R(1.4), R(1.0) + fexp(divc(u + R(5), R(5))));
+    const R Cd = dv(R(1.0), R(1.0) + fexp(divc(R(50) - u, R(20))));
     const R tau_d = Ad * Bd + Cd;
-    const R f_inf = R(1.0) / (R(1.0) + fexp(divc(u + R(20), R(7))));
+    const R f_inf = dv(R(1.0), R(1.0) + fexp(divc(u + R(20), R(7))));
     const R Af = R(1102.5) * fexp(divc(-(u + R(27)) * (u + R(27)), R(225)));
-    const R Bf = R(200.0) / (R(1) + fexp(divc(R(13) - u, R(10.0))));
-    const R Cf = R(180.0) / (R(1) + fexp(divc(u + R(30), R(10)))) + R(20);
+    const R Bf = dv(R(200.0), R(1) + fexp(divc(R(13) - u, R(10.0))));
+    const R Cf = dv(R(180.0), R(1) + fexp(divc(u + R(30), R(10)))) + R(20);
     const R tau_f = Af + Bf + Cf;
-    const R f2_inf = R(0.67) / (R(1.0) + fexp(divc(u + R(35), R(7)))) + R(0.33);
+    const R f2_inf = dv(R(0.67), R(1.0) + fexp(divc(u + R(35), R(7)))) + R(0.33);
     const R Af2 = R(600) * fexp(divc(-(u + R(25)) * (u + R(25)), R(170)));
-    const R Bf2 = R(31) / (R(1.0) + fexp(divc(R(25) - u, R(10))));
-    const R Cf2 = R(16) / (R(1.0) + fexp(divc(u + R(30), R(10))));
+    const R Bf2 = dv(R(31), R(1.0) + fexp(divc(R(25) - u, R(10))));
+    const R Cf2 = dv(R(16), R(1.0) + fexp(divc(u + R(30), R(10))));
     const R tau_f2 = Af2 + Bf2 + Cf2;
-    const R fcass_inf = R(0.6) / (R(1) + divc(cass, R(0.05)) * divc(cass, R(0.05))) + R(0.4);
-    const R tau_fcass = R(80.0) / (R(1) + divc(cass, R(0.05)) * divc(cass, R(0.05))) + R(2.0);
-    const R d = d_inf - (d_inf - dr) * fexp(-dt / tau_d);
-    const R f = f_inf - (f_inf - fr) * fexp(-dt / tau_f);
-    const R f2 = f2_inf - (f2_inf - f2r) * fexp(-dt / tau_f2);
-    const R fcass = fcass_inf - (fcass_inf - fcr) * fexp(-dt / tau_fcass);
+    const R fcass_inf = dv(R(0.6), R(1) + divc(cass, R(0.05)) * divc(cass, R(0.05))) + R(0.4);
+    const R tau_fcass = dv(R(80.0), R(1) + divc(cass, R(0.05)) * divc(cass, R(0.05))) + R(2.0);
+    const R d = d_inf - (d_inf - dr) * fexp(dv(-dt, tau_d));
+    const R f = f_inf - (f_inf - fr) * fexp(dv(-dt, tau_f));
+    const R f2 = f2_inf - (f2_inf - f2r) * fexp(dv(-dt, tau_f2));
+    const R fcass = fcass_inf - (fcass_inf - fcr) * fexp(dv(-dt, tau_fcass));
     dr = d;
     fr = f;
     f2r = f2;
@@ -565,8 +610,9 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
              (R(0.25) * fexp(X) * cass - cao) / (R(2) * F / (R_ * T) * exprel(X));
     } else {
       const R e = fexp(overRT(R(2) * (u - R(15)) * F));
-      ical = gcal * d * f * f2 * fcass * R(4) * (u - R(15)) * par<R>(P, kTp06Derived + kFFoverRT) *
-             (R(0.25) * e * cass - cao) / (e - R(1.0));
+      ical = dv(gcal * d * f * f2 * fcass * R(4) * (u - R(15)) *
+                    par<R>(P, kTp06Derived + kFFoverRT) * (R(0.25) * e * cass - cao),
+                e - R(1.0));
     }
   }
   // calc_ito
@@ -574,13 +620,13 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     auto&& rr_ = ref<R>(S, 11, c);
     auto&& sr = ref<R>(S, 12, c);
-    const R r_inf = R(1.0) / (R(1.0) + fexp(divc(R(20) - u, R(6.0))));
-    const R s_inf = R(1.0) / (R(1.0) + fexp(divc(u + R(20), R(5.0))));
+    const R r_inf = dv(R(1.0), R(1.0) + fexp(divc(R(20) - u, R(6.0))));
+    const R s_inf = dv(R(1.0), R(1.0) + fexp(divc(u + R(20), R(5.0))));
     const R tau_r = R(9.5) * fexp(divc(-(u + R(40.0)) * (u + R(40.0)), R(1800.0))) + R(0.8);
     const R tau_s = R(85.0) * fexp(divc(-(u + R(45.0)) * (u + R(45.0)), R(320.0))) +
-                    R(5.0) / (R(1.0) + fexp(divc(u - R(20.0), R(5.0)))) + R(3.0);
-    const R s = s_inf - (s_inf - sr) * fexp(-dt / tau_s);
-    const R r = r_inf - (r_inf - rr_) * fexp(-dt / tau_r);
+                    dv(R(5.0), R(1.0) + fexp(divc(u - R(20.0), R(5.0)))) + R(3.0);
+    const R s = s_inf - (s_inf - sr) * fexp(dv(-dt, tau_s));
+    const R r = r_inf - (r_inf - rr_) * fexp(dv(-dt, tau_r));
     sr = s;
     rr_ = r;
     ito = gto * r * s * (u - Ek);
@@ -590,16 +636,16 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     auto&& x1r = ref<R>(S, 8, c);
     auto&& x2r = ref<R>(S, 9, c);
-    const R xr1_inf = R(1.0) / (R(1.0) + fexp(divc(R(-26.0) - u, R(7.0))));
-    const R axr1 = R(450.0) / (R(1.0) + fexp(divc(R(-45.0) - u, R(10.0))));
-    const R bxr1 = R(6.0) / (R(1.0) + fexp(divc(u - R(-30.0), R(11.5))));
+    const R xr1_inf = dv(R(1.0), R(1.0) + fexp(divc(R(-26.0) - u, R(7.0))));
+    const R axr1 = dv(R(450.0), R(1.0) + fexp(divc(R(-45.0) - u, R(10.0))));
+    const R bxr1 = dv(R(6.0), R(1.0) + fexp(divc(u - R(-30.0), R(11.5))));
     const R tau_xr1 = axr1 * bxr1;
-    const R xr2_inf = R(1.0) / (R(1.0) + fexp(divc(u - R(-88.0), R(24.0))));
-    const R axr2 = R(3.0) / (R(1.0) + fexp(divc(R(-60.0) - u, R(20.0))));
-    const R bxr2 = R(1.12) / (R(1.0) + fexp(divc(u - R(60.0), R(20.0))));
+    const R xr2_inf = dv(R(1.0), R(1.0) + fexp(divc(u - R(-88.0), R(24.0))));
+    const R axr2 = dv(R(3.0), R(1.0) + fexp(divc(R(-60.0) - u, R(20.0))));
+    const R bxr2 = dv(R(1.12), R(1.0) + fexp(divc(u - R(60.0), R(20.0))));
     const R tau_xr2 = axr2 * bxr2;
-    const R xr1 = xr1_inf - (xr1_inf - x1r) * fexp(-dt / tau_xr1);
-    const R xr2 = xr2_inf - (xr2_inf - x2r) * fexp(-dt / tau_xr2);
+    const R xr1 = xr1_inf - (xr1_inf - x1r) * fexp(dv(-dt, tau_xr1));
+    const R xr2 = xr2_inf - (xr2_inf - x2r) * fexp(dv(-dt, tau_xr2));
     x1r = xr1;
     x2r = xr2;
     ikr = gkr * par<R>(P, kTp06Derived + kSqrtKo54) * xr1 * xr2 * (u - Ek);
@@ -608,34 +654,34 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   R iks;
   {
     auto&& xsr = ref<R>(S, 10, c);
-    const R Axs = R(1400.0) / fsqrt(R(1.0) + fexp(divc(R(5.0) - u, R(6))));
-    const R Bxs = R(1.0) / (R(1.0) + fexp(divc(u - R(35.0), R(15.0))));
+    const R Axs = dv(R(1400.0), fsqrt(R(1.0) + fexp(divc(R(5.0) - u, R(6)))));
+    const R Bxs = dv(R(1.0), R(1.0) + fexp(divc(u - R(35.0), R(15.0))));
     const R tau_xs = Axs * Bxs + R(80);
-    const R xs_inf = R(1.0) / (R(1.0) + fexp(divc(R(-5.0) - u, R(14.0))));
-    const R xs = xs_inf - (xs_inf - xsr) * fexp(-dt / tau_xs);
+    const R xs_inf = dv(R(1.0), R(1.0) + fexp(divc(R(-5.0) - u, R(14.0))));
+    const R xs = xs_inf - (xs_inf - xsr) * fexp(dv(-dt, tau_xs));
     xsr = xs;
     iks = gks * xs * xs * (u - Eks);
   }
   // calc_ik1
-  const R ak1 = R(0.1) / (R(1.0) + fexp(R(0.06) * (u - Ek - R(200))));
-  const R bk1 = (R(3.0) * fexp(R(0.0002) * (u - Ek + R(100))) +
-                 fexp(R(0.1) * (u - Ek - R(10)))) /
-                (R(1.0) + fexp(R(-0.5) * (u - Ek)));
-  const R rec_iK1 = ak1 / (ak1 + bk1);
+  const R ak1 = dv(R(0.1), R(1.0) + fexp(R(0.06) * (u - Ek - R(200))));
+  const R bk1 = dv(R(3.0) * fexp(R(0.0002) * (u - Ek + R(100))) +
+                      fexp(R(0.1) * (u - Ek - R(10))),
+                  R(1.0) + fexp(R(-0.5) * (u - Ek)));
+  const R rec_iK1 = dv(ak1, ak1 + bk1);
   const R ik1 = gk1 * rec_iK1 * (u - Ek);
   // calc_inaca
   const R e_n1 = fexp(overRT((n_ - R(1)) * u * F));
   const R inaca =
       knaca *
-      par<R>(P, kTp06Derived + kInvNaCaK) * par<R>(P, kTp06Derived + kInvKmCaCao) * (R(1.0) / (R(1) + ksat * e_n1)) *
+      par<R>(P, kTp06Derived + kInvNaCaK) * par<R>(P, kTp06Derived + kInvKmCaCao) * dv(R(1.0), R(1) + ksat * e_n1) *
       (fexp(overRT(n_ * u * F)) * nai * nai * nai * cao - e_n1 * nao * nao * nao * cai * R(2.5));
   // calc_inak
-  const R rec_iNaK = R(1.0) / (R(1.0) + R(0.1245) * fexp(overRT(R(-0.1) * u * F)) +
-                               R(0.0353) * fexp(overRT(-u * F)));
+  const R rec_iNaK = dv(R(1.0), R(1.0) + R(0.1245) * fexp(overRT(R(-0.1) * u * F)) +
+                                     R(0.0353) * fexp(overRT(-u * F)));
   const R inak =
-      knak * par<R>(P, kTp06Derived + kKoOverKoKmK) * (nai / (nai + KmNa)) * rec_iNaK;
-  const R ipca = gpca * cai / (KpCa + cai);
-  const R rec_ipK = R(1.0) / (R(1.0) + fexp(divc(R(25) - u, R(5.98))));
+      knak * par<R>(P, kTp06Derived + kKoOverKoKmK) * dv(nai, nai + KmNa) * rec_iNaK;
+  const R ipca = dv(gpca * cai, KpCa + cai);
+  const R rec_ipK = dv(R(1.0), R(1.0) + fexp(divc(R(25) - u, R(5.98))));
   const R ipk = gpk * rec_ipK * (u - Ek);
   const R ibna = gbna * (u - Ena);
   const R ibca = gbca * (u - Eca);
@@ -644,24 +690,24 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     auto&& rrr = ref<R>(S, 17, c);
     auto&& oor = ref<R>(S, 18, c);
-    const D kCaSR = maxsr - (maxsr - minsr) / (1 + EC / casrD * (EC / casrD));
-    const D k1 = k1_ / kCaSR;
+    const D kCaSR = maxsr - dv(maxsr - minsr, 1 + dv(EC, casrD) * dv(EC, casrD));
+    const D k1 = dv(k1_, kCaSR);
     const D k2 = k2_ * kCaSR;
     D rr = (D)rrr;
     const D drr = k4 * (1 - rr) - k2 * cassD * rr;
     rr += (D)dt * drr;
-    const D oo = k1 * cassD * cassD * rr / (k3 + k1 * cassD * cassD);
+    const D oo = dv(k1 * cassD * cassD * rr, k3 + k1 * cassD * cassD);
     rrr = (R)rr;
     oor = (R)oo;
     irel = Vrel * oo * (casrD - cassD);
   }
   const D ileak = Vleak * (casrD - caiD);
-  const D iup = Vmaxup / (1.0 + Kup * Kup / (caiD * caiD));
+  const D iup = dv(Vmaxup, 1.0 + dv(Kup * Kup, caiD * caiD));
   const D ixfer = Vxfer * (cassD - caiD);
   const D dtD = (D)dt;
   // calc_casr: closed-form buffer quadratic
   {
-    const D CaCSQN = Bufsr * casrD / (casrD + Kbufsr);
+    const D CaCSQN = dv(Bufsr * casrD, casrD + Kbufsr);
     const D dCaSR = dtD * (iup - irel - ileak);
     const D bjsr = Bufsr - CaCSQN - dCaSR - casrD + Kbufsr;
     const D cjsr = Kbufsr * (CaCSQN + dCaSR + casrD);
@@ -669,7 +715,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   }
   // calc_cass
   {
-    const D CaSSBuf = Bufss * cassD / (cassD + Kbufss);
+    const D CaSSBuf = dv(Bufss * cassD, cassD + Kbufss);
     const D dCaSS = dtD * (-ixfer * Dv[kVcOverVss] + irel * Dv[kVsrOverVss] +
                            -(D)ical * inversevssF2 * CAPACITANCE);
     const D bcss = Bufss - CaSSBuf - dCaSS - cassD + Kbufss;
@@ -681,10 +727,10 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     const D dNai = -((D)ina + (D)ibna + 3 * (D)inak + 3 * (D)inaca) * inverseVcF *
                    CAPACITANCE;
-    nair = naiD + dtD * dNai;
+    nair = (Sc)((F64 ? naiD : naiS) + dtD * dNai);
     const D dKi = -((D)ik1 + (D)ito + (D)ikr + (D)iks - 2 * (D)inak + (D)ipk) *
                   inverseVcF * CAPACITANCE;
-    Kir = KiD + dtD * dKi;
+    Kir = (Sc)((F64 ? KiD : KiS) + dtD * dKi);
   }
   return un - dt * (ikr + iks + ik1 + ito + ina + ibna + ical + ibca + inak +
                     inaca + ipca + ipk);
@@ -1012,10 +1058,10 @@ __host__ __device__ constexpr int model_n_params(int model) {
        : model == 4 ? 28 : model == 5 ? 15 : model == 6 ? 49 : 39;
 }
 // FP64 storage of state k under precision prec (1 = FP32 path)
-// TP06 on the FP32 path keeps cai, casr, cass, nai, Ki (k < 5) and the
-// sodium inactivation gates h, j (k = 6, 7) double: see rl_gate
+// TP06 on the FP32 path keeps its calcium concentrations cai, casr, cass
+// (k < 3) double; nai, Ki, h and j are float differences (state_offset)
 __host__ __device__ constexpr bool state_is_f64(int model, int prec, int k) {
-  return prec == 0 ? true : (model == 6 && (k < 5 || k == 6 || k == 7));
+  return prec == 0 ? true : (model == 6 && k < 3);
 }
 
 }  // namespace fw

@@ -199,6 +199,8 @@ class Engine:
         self.state_names = model._state_call_order()
         self.state_f64 = [_native.state_is_f64(self.model_id, self.precision, k)
                           for k in range(len(self.state_names))]
+        self.state_off = [_native.state_offset(self.model_id, self.precision, k)
+                          for k in range(len(self.state_names))]
         if self.stim_sched:
             self._push_stims()
 
@@ -252,10 +254,13 @@ class Engine:
         self.B = torch.from_numpy(np.ascontiguousarray(m.u_new)).to(self.dev, self.rdtype)
         self.swapped = False
         self.state = []
-        for name, f64 in zip(self.state_names, self.state_f64):
+        for name, f64, off in zip(self.state_names, self.state_f64, self.state_off):
             dt = torch.float64 if f64 else torch.float32
+            a = getattr(m, name)
+            if off:  # stored as the difference from `off`, formed in double
+                a = np.asarray(a, dtype=np.float64) - off
             self.state.append(torch.from_numpy(
-                np.ascontiguousarray(getattr(m, name))).to(self.dev, dt).contiguous())
+                np.ascontiguousarray(a)).to(self.dev, dt).contiguous())
         self._state_ptrs = (c_void_p * max(1, len(self.state)))(
             *[s.data_ptr() for s in self.state])
 
@@ -269,8 +274,9 @@ class Engine:
         m = self.model
         m.u = self.u_dev().cpu().numpy()
         m.u_new = self.u_new_dev().cpu().numpy()
-        for name, t in zip(self.state_names, self.state):
-            setattr(m, name, t.cpu().numpy())
+        for name, t, off in zip(self.state_names, self.state, self.state_off):
+            a = t.cpu().numpy()
+            setattr(m, name, a.astype(np.float64) + off if off else a)
 
     # -- stepping ------------------------------------------------------------
     def run_steps(self, step0: int, n: int, act_tracker=None, act_flags=None,

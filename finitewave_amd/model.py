@@ -127,9 +127,13 @@ class CardiacModel:
         u = torch.from_numpy(np.ascontiguousarray(self.u)).to(dev, rd)
         names = self._state_call_order()
         st = []
+        offs = [_native.state_offset(mid, prec, k) for k in range(len(names))]
         for k, n in enumerate(names):
             dt = torch.float64 if _native.state_is_f64(mid, prec, k) else torch.float32
-            st.append(torch.from_numpy(np.ascontiguousarray(getattr(self, n))).to(dev, dt))
+            a = getattr(self, n)
+            if offs[k]:  # stored as the difference from offs[k]
+                a = np.asarray(a, dtype=np.float64) - offs[k]
+            st.append(torch.from_numpy(np.ascontiguousarray(a)).to(dev, dt))
         idx = torch.from_numpy(np.ascontiguousarray(self.cardiac_tissue.myo_indexes,
                                                     dtype=np.int64)).to(dev)
         import ctypes
@@ -141,8 +145,8 @@ class CardiacModel:
                                              float(self.dt), par, len(params), _stream()),
                       "fw_ionic")
         self.u_new[...] = un.cpu().numpy()
-        for n, s in zip(names, st):
-            getattr(self, n)[...] = s.cpu().numpy()
+        for n, s, off in zip(names, st, offs):
+            getattr(self, n)[...] = s.cpu().numpy().astype(np.float64) + off if off else s.cpu().numpy()
 
     def check_termination(self):
         # cardiac_model.py:191-203

@@ -69,7 +69,7 @@
 extern "C" {
 #endif
 
-#define FW_ABI_VERSION 3
+#define FW_ABI_VERSION 4
 
 /* status codes */
 #define FW_OK 0
@@ -97,8 +97,9 @@ extern "C" {
 
 /* precision */
 #define FW_PREC_FP64 0 /* all state FP64; bit-for-bit the reference order */
-#define FW_PREC_FP32 1 /* FP32 state; TP06 cai/casr/cass/nai/Ki and the potential of
-                          LR91/TP06/Courtemanche kept FP64 (fw_u_is_f64) */
+#define FW_PREC_FP32 1 /* FP32 state; TP06 cai/casr/cass and the potential of
+                          LR91/TP06/Courtemanche kept FP64 (fw_u_is_f64); TP06
+                          nai/Ki/h/j stored as differences (fw_state_offset) */
 
 /* half-step diffusion field storage */
 #define FW_FIELD_UNIFORM 0 /* one value */
@@ -181,6 +182,13 @@ const char* fw_last_error(void);
  * stored as FP64 under `precision` (1) or FP32 (0). */
 int fw_model_info(int32_t model, int32_t* n_state, int32_t* n_params);
 int fw_state_is_f64(int32_t model, int32_t precision, int32_t k);
+/* The value state k is stored relative to under `precision`: a state array
+ * holds value - *offset.  0 except on the FP32 path of TP06, which stores
+ * nai, Ki, h and j (k = 3, 4, 6, 7) as float differences from their initial
+ * values 7.67, 138.3, 0.75, 0.75 (a float holding the value itself stops
+ * following their slow changes).  Applies to every entry point that takes
+ * state arrays (fw_ionic, fw_run, batches); probes record values. */
+int fw_state_offset(int32_t model, int32_t precision, int32_t k, double* offset);
 /* Whether u / u_new are FP64 (1) or FP32 (0) under `precision`: FP64 on the
  * FP64 path, and on the FP32 path for LR91, TP06 and Courtemanche, whose
  * resting potential a float cannot follow (increments below half an ulp near

@@ -28,10 +28,15 @@ def test_library_loads_and_exports_every_declared_symbol():
     assert L.fw_abi_version() == _native.ABI_VERSION
     assert _native.model_info(_native.MODEL_IDS["tp06"]) == (19, 49)
     assert _native.model_info(_native.MODEL_IDS["courtemanche"]) == (21, 39)
-    # TP06's cai/casr/cass/nai/Ki and sodium gates h, j stay FP64 on the FP32 path
-    assert [_native.state_is_f64(6, 1, k) for k in range(9)] == \
-        [True] * 5 + [False, True, True, False]
+    # TP06's calcium concentrations cai/casr/cass stay FP64 on the FP32 path;
+    # nai, Ki and the sodium gates h, j are float differences from 7.67,
+    # 138.3, 0.75, 0.75 (the initial values)
+    assert [_native.state_is_f64(6, 1, k) for k in range(9)] == [True] * 3 + [False] * 6
     assert all(_native.state_is_f64(6, 0, k) for k in range(19))
+    assert [_native.state_offset(6, 1, k) for k in range(9)] == \
+        [0.0, 0.0, 0.0, 7.67, 138.3, 0.0, 0.75, 0.75, 0.0]
+    assert all(_native.state_offset(m, p, k) == 0.0 for m in range(8) for p in (0, 1)
+               for k in range(_native.model_info(m)[0]) if (m, p) != (6, 1))
     # the potential: FP64 on the FP64 path and for LR91/TP06/CRN on the FP32 path
     assert [_native.u_is_f64(m, 1) for m in range(8)] == [False] * 5 + [True] * 3
     assert all(_native.u_is_f64(m, 0) for m in range(8))
