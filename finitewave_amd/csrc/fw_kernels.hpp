@@ -185,17 +185,19 @@ struct KArgs {
 };
 
 // Lane shifts across the whole 64-lane wavefront by DPP (v_mov_b32_dpp
-// wave_shr:1 / wave_shl:1): a VALU move, no LDS round trip.
+// wave_shr:1 / wave_shl:1): a VALU move, no LDS round trip.  The bound-control
+// bit writes 0 to the lane without a source, so no destination needs
+// clearing first.
 // lane i receives lane i-1's value (lane 0 receives 0)
 template <typename T>
 __device__ __forceinline__ T from_west(T x) {
   if constexpr (sizeof(T) == 4) {
     return __builtin_bit_cast(
-        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int32_t, x), 0x138, 0xf, 0xf, false));
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int32_t, x), 0x138, 0xf, 0xf, true));
   } else {
     const int64_t b = __builtin_bit_cast(int64_t, x);
-    const int32_t lo = __builtin_amdgcn_update_dpp(0, (int32_t)b, 0x138, 0xf, 0xf, false);
-    const int32_t hi = __builtin_amdgcn_update_dpp(0, (int32_t)(b >> 32), 0x138, 0xf, 0xf, false);
+    const int32_t lo = __builtin_amdgcn_update_dpp(0, (int32_t)b, 0x138, 0xf, 0xf, true);
+    const int32_t hi = __builtin_amdgcn_update_dpp(0, (int32_t)(b >> 32), 0x138, 0xf, 0xf, true);
     return __builtin_bit_cast(T, ((int64_t)hi << 32) | (int64_t)(uint32_t)lo);
   }
 }
@@ -205,11 +207,11 @@ template <typename T>
 __device__ __forceinline__ T from_east(T x) {
   if constexpr (sizeof(T) == 4) {
     return __builtin_bit_cast(
-        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int32_t, x), 0x130, 0xf, 0xf, false));
+        T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int32_t, x), 0x130, 0xf, 0xf, true));
   } else {
     const int64_t b = __builtin_bit_cast(int64_t, x);
-    const int32_t lo = __builtin_amdgcn_update_dpp(0, (int32_t)b, 0x130, 0xf, 0xf, false);
-    const int32_t hi = __builtin_amdgcn_update_dpp(0, (int32_t)(b >> 32), 0x130, 0xf, 0xf, false);
+    const int32_t lo = __builtin_amdgcn_update_dpp(0, (int32_t)b, 0x130, 0xf, 0xf, true);
+    const int32_t hi = __builtin_amdgcn_update_dpp(0, (int32_t)(b >> 32), 0x130, 0xf, 0xf, true);
     return __builtin_bit_cast(T, ((int64_t)hi << 32) | (int64_t)(uint32_t)lo);
   }
 }

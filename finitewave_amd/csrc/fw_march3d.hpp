@@ -137,19 +137,15 @@ __global__ __launch_bounds__(kBlock) void k_step3_march(const KArgs<float, float
   float fx_lo = 0.0f;
   Qv xy_lo{0.0f, 0.0f}, xym_lo{0.0f, 0.0f}, xz_lo{0.0f, 0.0f};
   for (int i = ibeg; i < i1; ++i, c += si) {
-    // plane i + 1 (none after the last plane)
-    float n0 = u1, n1 = u1, n2 = u1, n0p = u1, n1p = u1, n2p = u1;
-    float q0 = 0.0f, q1 = 0.0f, q2 = 0.0f, q0p = 0.0f, q1p = 0.0f, q2p = 0.0f;
-    uint8_t cn1 = 0;
-    if (i + 1 < a.ni) {
-      const int64_t cn = c + si;
-      n0 = u[cn + om], n1 = u[cn], n2 = u[cn + op];
-      n0p = u[cn + om + ok], n1p = u[cn + ok], n2p = u[cn + op + ok];
-      cn1 = code[cn];
-      q0 = (float)(code[cn + om] & 1), q1 = (float)(cn1 & 1), q2 = (float)(code[cn + op] & 1);
-      q0p = (float)(code[cn + om + ok] & 1), q1p = (float)(code[cn + ok] & 1);
-      q2p = (float)(code[cn + op + ok] & 1);
-    }
+    // plane i + 1; the last plane (i = ni - 1) takes itself again: no node
+    // of it is updated and no plane follows, so its fluxes are not used
+    const int64_t cn = i + 1 < a.ni ? c + si : c;
+    const float n0 = u[cn + om], n1 = u[cn], n2 = u[cn + op];
+    const float n0p = u[cn + om + ok], n1p = u[cn + ok], n2p = u[cn + op + ok];
+    const uint8_t cn1 = code[cn];
+    const float q0 = (float)(code[cn + om] & 1), q1 = (float)(cn1 & 1), q2 = (float)(code[cn + op] & 1);
+    const float q0p = (float)(code[cn + om + ok] & 1), q1p = (float)(code[cn + ok] & 1);
+    const float q2p = (float)(code[cn + op + ok] & 1);
     // the squares this node starts: S_xy(c), S_xy(c-y), S_xz(c), S_yz(c), S_yz(c-y)
     const Qv xy = qv(square4(m1, u1, q1, n1, m2, u2, q2, n2));
     const Qv xym = qv(square4(m0, u0, q0, n0, m1, u1, q1, n1));

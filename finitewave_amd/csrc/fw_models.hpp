@@ -512,10 +512,11 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   auto&& cassr = ref<D>(S, 2, c);
   auto&& nair = ref<Sc>(S, 3, c);
   auto&& Kir = ref<Sc>(S, 4, c);
-  const D caiD = cair, casrD = casrr, cassD = cassr;
-  const D naiS = (D)(Sc)nair, KiS = (D)(Sc)Kir;
-  const D naiD = F64 ? naiS : naiS + kNai0, KiD = F64 ? KiS : KiS + kKi0;
-  const R cai = (R)caiD, cass = (R)cassD, nai = (R)naiD, Ki = (R)KiD;
+  // read where they are used: the balances at the end read them again
+  // rather than keep them live across the currents
+  const R cai = (R)(D)cair, cass = (R)(D)cassr;
+  const R nai = (R)(F64 ? (D)(Sc)nair : (D)(Sc)nair + kNai0);
+  const R Ki = (R)(F64 ? (D)(Sc)Kir : (D)(Sc)Kir + kKi0);
 
   const R Ek = RTONF * flog(dv(ko, Ki));
   const R Ena = RTONF * flog(dv(nao, nai));
@@ -690,6 +691,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     auto&& rrr = ref<R>(S, 17, c);
     auto&& oor = ref<R>(S, 18, c);
+    const D casrD = casrr, cassD = cassr;
     const D kCaSR = maxsr - dv(maxsr - minsr, 1 + dv(EC, casrD) * dv(EC, casrD));
     const D k1 = dv(k1_, kCaSR);
     const D k2 = k2_ * kCaSR;
@@ -701,6 +703,7 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
     oor = (R)oo;
     irel = Vrel * oo * (casrD - cassD);
   }
+  const D caiD = cair, casrD = casrr, cassD = cassr;
   const D ileak = Vleak * (casrD - caiD);
   const D iup = dv(Vmaxup, 1.0 + dv(Kup * Kup, caiD * caiD));
   const D ixfer = Vxfer * (cassD - caiD);
@@ -727,10 +730,10 @@ __device__ __forceinline__ U ionic_tp06(U u_in, U un, SP& S, int64_t c, R dt,
   {
     const D dNai = -((D)ina + (D)ibna + 3 * (D)inak + 3 * (D)inaca) * inverseVcF *
                    CAPACITANCE;
-    nair = (Sc)((F64 ? naiD : naiS) + dtD * dNai);
+    nair = (Sc)((D)(Sc)nair + dtD * dNai);
     const D dKi = -((D)ik1 + (D)ito + (D)ikr + (D)iks - 2 * (D)inak + (D)ipk) *
                   inverseVcF * CAPACITANCE;
-    Kir = (Sc)((F64 ? KiD : KiS) + dtD * dKi);
+    Kir = (Sc)((D)(Sc)Kir + dtD * dKi);
   }
   return un - dt * (ikr + iks + ik1 + ito + ina + ibna + ical + ibca + inak +
                     inaca + ipca + ipk);
