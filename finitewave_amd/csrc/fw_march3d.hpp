@@ -71,13 +71,17 @@ struct Qv {
   float ok;  // 1 or 0
 };
 
-__device__ __forceinline__ Qv qv(Sq s) { return Qv{qmean(s), s.m > 2.5f ? 1.0f : 0.0f}; }
+// ok = [m >= 3] for counts m in {0, .., 4}: m - 2 clamped to [0, 1]
+__device__ __forceinline__ Qv qv(Sq s) {
+  return Qv{qmean(s), fminf(fmaxf(s.m - 2.0f, 0.0f), 1.0f)};
+}
 
-// a cross term of the face (c, c+a) between its two squares; mf = m(c) m(c+a)
+// a cross term of the face (c, c+a) between its two squares; mf = m(c) m(c+a).
+// The flags are exact 0s and 1s: the product is d (hi.q - lo.q) when the
+// face and both squares qualify and 0 otherwise, without a select.
 __device__ __forceinline__ float cross_term(float d, float mf, Qv hi, Qv lo) {
 #pragma clang fp contract(fast)
-  const bool on = mf * hi.ok * lo.ok > 0.5f;
-  return on ? d * (hi.q - lo.q) : 0.0f;
+  return (d * mf) * (hi.ok * lo.ok) * (hi.q - lo.q);
 }
 
 // lane - 1's square by a DPP wavefront shift (lane 0 receives zeros; its
