@@ -51,8 +51,11 @@ FW_HD inline double f64_from(uint64_t u) { return __builtin_bit_cast(double, u);
 
 // exp(x): the reduction and polynomial below run for every argument; a
 // wavefront branches to the fix-up of t128_exp_special only when one of its
-// lanes has |x| < 2^-54 or |x| >= 512 (or inf, NaN).  The fix-up is inlined: a
-// call would make the caller keep its live registers across 70-odd call sites.
+// lanes has |x| >= 512 (or inf, NaN): one compare of |x| per call.  The fix-up
+// is inlined: a call would make the caller keep its live registers across
+// 70-odd call sites.  Tiny arguments need no fix-up: for |x| < 2^-54 the
+// reduction gives k = 0 and r = x, the polynomial's terms beyond r vanish
+// against x, and the result is fma(1, x, 1) = RN(1 + x), the fix-up's value.
 FW_HD inline double t128_exp_special(double x, double tmp, uint64_t sbits, uint64_t ki) {
   const uint64_t ux = f64_bits(x);
   const uint32_t abstop = (uint32_t)(ux >> 52) & 0x7ffu;
@@ -82,7 +85,6 @@ FW_HD inline double t128_exp_special(double x, double tmp, uint64_t sbits, uint6
 }
 
 FW_HD inline double t128_exp(double x) {
-  const uint32_t abstop = (uint32_t)(f64_bits(x) >> 52) & 0x7ffu;
   double kd = fma(mt::kInvLn2N, x, mt::kShift);
   const uint64_t ki = f64_bits(kd);
   kd -= mt::kShift;
@@ -93,8 +95,7 @@ FW_HD inline double t128_exp(double x) {
   const double r2 = r * r;
   const double tmp =
       fma(r2 * r2, fma(r, mt::kC5, mt::kC4), fma(r2, fma(r, mt::kC3, mt::kC2), tail + r));
-  // top12(2^-54) = 0x3c9, top12(512) = 0x408
-  if (__builtin_expect(abstop - 0x3c9u >= 0x408u - 0x3c9u, 0))
+  if (__builtin_expect(!(__builtin_fabs(x) < 512.0), 0))  // |x| >= 512, inf, NaN
     return t128_exp_special(x, tmp, sbits, ki);
   const double scale = f64_from(sbits);
   return fma(scale, tmp, scale);
