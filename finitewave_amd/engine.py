@@ -270,13 +270,19 @@ class Engine:
     def u_new_dev(self):
         return self.A if self.swapped else self.B
 
+    def state_values(self):
+        """(name, host array) of every state array as the model holds it: the
+        value, i.e. the stored difference plus its offset where there is one."""
+        for name, t, off in zip(self.state_names, self.state, self.state_off):
+            a = t.cpu().numpy()
+            yield name, (a.astype(np.float64) + off if off else a)
+
     def download(self) -> None:
         m = self.model
         m.u = self.u_dev().cpu().numpy()
         m.u_new = self.u_new_dev().cpu().numpy()
-        for name, t, off in zip(self.state_names, self.state, self.state_off):
-            a = t.cpu().numpy()
-            setattr(m, name, a.astype(np.float64) + off if off else a)
+        for name, a in self.state_values():
+            setattr(m, name, a)
 
     # -- stepping ------------------------------------------------------------
     def run_steps(self, step0: int, n: int, act_tracker=None, act_flags=None,

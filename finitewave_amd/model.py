@@ -396,8 +396,8 @@ class CardiacModel:
         m = self
         m.u = eng.u_dev().cpu().numpy()
         m.u_new = eng.u_new_dev().cpu().numpy()
-        for name, t in zip(eng.state_names, eng.state):
-            setattr(m, name, t.cpu().numpy())
+        for name, a in eng.state_values():
+            setattr(m, name, a)
 
     def _after_host_change(self, eng) -> None:
         """A command may rewrite state, mesh, fibers or weights.  A command

@@ -343,3 +343,59 @@ def test_field_layout_kernels_equal_general_kernel(gpu, name, fibres, precision)
     assert np.array_equal(u_a, u_b)
     for n in st_a:
         assert np.array_equal(st_a[n], st_b[n]), n
+
+
+def _tp06_run(precision, t_stops, track_vars=None, cell=(10, 10)):
+    """A 24x24 TP06 tissue stimulated along one edge, run to each time in
+    t_stops in turn (the first call initializes, the others continue)."""
+    shape = (24, 24)
+    model = fw.TP062D()
+    model.dt, model.dr = 0.01, 0.25
+    model.precision = precision
+    model.cardiac_tissue = fw.CardiacTissue2D(list(shape))
+    model.prog_bar = False
+    seq = fw.StimSequence()
+    seq.add_stim(fw.StimCurrentCoord2D(0, 52, 1.0, 0, shape[0], 0, 4))
+    model.stim_sequence = seq
+    tracker = None
+    if track_vars:
+        tracker = fw.MultiVariable2DTracker()
+        tracker.cell_ind = list(cell)
+        tracker.var_list = list(track_vars)
+        ts = fw.TrackerSequence()
+        ts.add_tracker(tracker)
+        model.tracker_sequence = ts
+    for n, t in enumerate(t_stops):
+        model.t_max = t
+        model.run(initialize=(n == 0))
+    return model, tracker
+
+
+@pytest.mark.gpu
+def test_fp32_tp06_stored_differences_read_back_as_values(gpu):
+    """The FP32 path keeps TP06's nai, Ki, h and j on the device as float
+    differences from their initial values (fw_state_offset).  Every host view
+    holds values: the attributes after run(), the device probes of a variable
+    tracker; a run continued with run(initialize=False) re-encodes them and
+    follows the uninterrupted run; and the values stay within FP32 accuracy of
+    the FP64 path.  A missing or doubled offset moves nai by 7.67, Ki by 138.3
+    and h, j by 0.75."""
+    names = ("nai", "Ki", "h", "j")
+    cell = (10, 10)
+    whole, tracker = _tp06_run("fp32", [6.0], track_vars=names, cell=cell)
+    split, _ = _tp06_run("fp32", [2.0, 6.0])
+    ref, _ = _tp06_run("fp64", [6.0])
+    assert np.max(whole.u[cell]) > 0.0  # the wave has reached the probed node
+    for n in names:
+        a, b, r = getattr(whole, n), getattr(split, n), getattr(ref, n)
+        assert a.dtype == np.float64, n
+        # re-encoding a value rounds only differences far below a float's
+        # resolution of the value (|d| < 2^-22 of the offset)
+        assert np.max(np.abs(a - b)) <= 1e-9 * max(1.0, float(np.max(np.abs(r)))), n
+        # at the front a timing difference well inside one step moves the
+        # fast sodium gates by up to a few hundredths
+        tol = {"nai": 1e-4, "Ki": 1e-4, "h": 0.05, "j": 0.05}[n]
+        assert np.max(np.abs(a - r)) <= tol, (n, float(np.max(np.abs(a - r))))
+        # the probe records the value the attribute ends with
+        assert tracker.output[n][-1] == a[cell], n
+    assert np.allclose(whole.u, split.u, rtol=0, atol=1e-9)
