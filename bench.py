@@ -64,6 +64,9 @@ B_STATE = {"fp64": 304, "fp32": 172}
 # holding differences from their initial values
 B_STORED = {"fp64": 304, "fp32": 180}
 B_TRACK = 4
+# the serving leg runs TP06 tissues at or above this fibrotic fraction on the
+# FP64 path (activation times within one step of the FP64 path throughout)
+FP64_FIBROSIS = 0.3
 
 
 def profiled_traffic(key: str):
@@ -592,23 +595,34 @@ def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False,
     FP64 path (its sims/s is that sample's rate)."""
     import torch
     from finitewave_amd.analysis import apd90
-    from finitewave_amd.serving import Request, Server
+    from finitewave_amd.serving import Request, Server, fp64_for_fibrosis
     from finitewave_amd.workloads import serving_stream
     classes, specs = serving_stream(n=n_req)
     reqs = [Request(rid=k, arrival=s["arrival"], model=s["model"], sim=s["sim"])
             for k, s in enumerate(specs)]
-    srv = Server(classes, precision="fp32", max_tissues=64, chunk_steps=1000)
-    stats = srv.serve(reqs, realtime=True)
-    shapes = {k: dict(variant=b.variant, tuning_ms={str(v): round(t, 4) for v, t in
-                                                   (b.tuning or {}).items()})
-              for k, b in srv.batches.items()}
+    # FP32, except TP06 tissues close to conduction failure: there float
+    # rounding moves activation times by more than a step
+    policy = fp64_for_fibrosis(("tp06",), FP64_FIBROSIS)
+    srv = Server(classes, precision="fp32", max_tissues=64, chunk_steps=1000,
+                 precision_for=policy)
+
+    def served(r):  # progress: one line per finished request
+        print(f"# serving: request {r.rid} ({r.model}, {r.precision}) {r.status} at "
+              f"{r.finished:.1f} s", file=sys.stderr, flush=True)
+    stats = srv.serve(reqs, realtime=True, log=served)
+    shapes = {(m if p == "fp32" else f"{m}_{p}"): dict(
+                  variant=b.variant,
+                  tuning_ms={str(v): round(t, 4) for v, t in (b.tuning or {}).items()})
+              for (m, p), b in srv.batches.items()}
     srv.close()
     cell_steps = sum(int(np.count_nonzero(r.sim.tissue.mesh == 1)) * r.sim.n_steps for r in reqs)
     out = dict(workload=f"{n_req} requests, Poisson 200/s, TP06 or Aliev-Panfilov 2D, side "
                         "U{128..1024}+2, fibrosis p~U[0,0.4], fibre U[0,180) deg (9-point), "
                         "planar or S1-S2 cross-field, horizon U[500,2000] ms, FP32 (TP06 Ca "
-                        "FP64)",
+                        "FP64); TP06 tissues with fibrotic fraction >= "
+                        f"{FP64_FIBROSIS} on the FP64 path",
                requests=n_req, sims_per_s=round(stats["sims_per_s"], 4),
+               fp64_requests=sum(r.precision == "fp64" for r in reqs),
                p50_s=round(stats["p50_s"], 3), p99_s=round(stats["p99_s"], 3),
                wall_s=round(stats["wall_s"], 2),
                gcell=round(cell_steps / stats["wall_s"] / 1e9, 3),
@@ -626,6 +640,8 @@ def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False,
             act64, trace64 = _fp64_plain(classes[r.model], r.sim)
             torch.cuda.synchronize()
             t_plain += time.perf_counter() - w0
+            print(f"# fp64 plain: request {r.rid} ({r.model}) in "
+                  f"{time.perf_counter() - w0:.1f} s", file=sys.stderr, flush=True)
             a32 = r.sim.result["act_t"]
             both = (a32 >= 0) & (act64 >= 0)
             d = np.abs(a32[both] - act64[both])
@@ -638,7 +654,8 @@ def serving_leg(n_req: int, fp64_baseline: bool, per_request: bool = False,
             if a is not None and a64 is not None:
                 apd_err.append(abs(a - a64))
             tag = r.sim.tag
-            per_req.append(dict(model=r.model, side=tag["side"], p=round(tag["p"], 3),
+            per_req.append(dict(model=r.model, precision=r.precision, side=tag["side"],
+                                p=round(tag["p"], 3),
                                 cross=tag["cross"], lat_max_ms=float(d.max()) if d.size else 0.0,
                                 lat_p99_ms=float(np.percentile(d, 99)) if d.size else 0.0,
                                 over_1dt=int(np.count_nonzero(d > 0.01 + 1e-9)),

@@ -9,6 +9,12 @@ chunks of steps, retires finished tissues (their activation map, probe trace
 and APD90 are read back) and records each request's latency from arrival to
 results.
 
+A precision policy (``precision_for``) may route a request to a batch of
+another precision: ``fp64_for_fibrosis`` serves the physiological models'
+tissues whose fibrotic fraction reaches a threshold on the FP64 path, where
+conduction close to failure lets float rounding move activation times by
+more than a step.
+
 A request ends in one of three states: ``done`` (results read back),
 ``cancelled`` (``Server.cancel(rid)`` or its ``deadline_s`` passed: it
 leaves the queue, or its tissue leaves its batch at the next launch
@@ -31,8 +37,9 @@ from .batch import BatchSim, TissueBatch
 class Request:
     rid: int
     arrival: float          # seconds after the start of serving
-    model: str              # key of the server's batches
+    model: str              # key of the server's model classes
     sim: BatchSim
+    precision: str | None = None      # set by the server: "fp32" or "fp64"
     footprint: int = 0
     admitted: float | None = None
     finished: float | None = None
@@ -45,15 +52,37 @@ class Request:
         return None if self.finished is None else self.finished - self.arrival
 
 
+def fibrotic_fraction(sim: BatchSim) -> float:
+    """Fraction of the tissue's interior nodes that do not conduct."""
+    inner = np.asarray(sim.tissue.mesh)[1:-1, 1:-1]
+    return float(np.count_nonzero(inner != 1)) / max(1, inner.size)
+
+
+def fp64_for_fibrosis(models=("tp06",), threshold: float = 0.3, base: str = "fp32"):
+    """A precision policy: requests of `models` whose tissue's fibrotic
+    fraction is at least `threshold` run on the FP64 path, the rest on `base`."""
+    models = tuple(models)
+
+    def policy(req: "Request") -> str:
+        if req.model in models and fibrotic_fraction(req.sim) >= threshold:
+            return "fp64"
+        return base
+    return policy
+
+
 class Server:
     def __init__(self, model_classes: dict, precision: str = "fp32", dt: float = 0.01,
                  dr: float = 0.25, hbm_budget: int | None = None, max_tissues: int = 256,
-                 chunk_steps: int = 200, use_graph: bool = True, check_every: int = 1):
+                 chunk_steps: int = 200, use_graph: bool = True, check_every: int = 1,
+                 precision_for=None):
         import torch
-        self.batches = {k: TissueBatch(cls, precision=precision, dt=dt, dr=dr,
-                                       max_tissues=max_tissues, use_graph=use_graph,
-                                       chunk_cap=chunk_steps)
-                        for k, cls in model_classes.items()}
+        self.model_classes = dict(model_classes)
+        self.precision = precision
+        self.precision_for = precision_for
+        self._batch_args = dict(dt=dt, dr=dr, max_tissues=max_tissues, use_graph=use_graph,
+                                chunk_cap=chunk_steps)
+        # one batch per (model, precision), made when a request first needs it
+        self.batches: dict = {}
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info()
             hbm_budget = int(0.6 * free)
@@ -73,8 +102,19 @@ class Server:
         the next launch boundary; results are not read."""
         self._cancel.add(rid)
 
+    def batch(self, req: Request) -> TissueBatch:
+        """The batch `req` runs in (its model and precision)."""
+        if req.precision is None:
+            req.precision = (self.precision_for(req) if self.precision_for
+                             else self.precision)
+        key = (req.model, req.precision)
+        if key not in self.batches:
+            self.batches[key] = TissueBatch(self.model_classes[req.model],
+                                            precision=req.precision, **self._batch_args)
+        return self.batches[key]
+
     def footprint(self, req: Request) -> int:
-        b = self.batches[req.model]
+        b = self.batch(req)
         steps = int(np.ceil(req.sim.t_max / b.dt))
         return b.footprint(np.shape(req.sim.tissue.mesh), steps)
 
@@ -113,14 +153,14 @@ class Server:
                 end(r, "cancelled")
             for key, r in list(active.items()):
                 if expired(r, now):
-                    self.batches[r.model].drop(r.sim.slot)
+                    self.batch(r).drop(r.sim.slot)
                     del active[key]
                     self.in_use -= r.footprint
                     end(r, "cancelled")
             # admission
             while pending and (not realtime or pending[0].arrival <= now):
                 r = pending[0]
-                b = self.batches[r.model]
+                b = self.batch(r)
                 if self.in_use + r.footprint > self.hbm_budget or not b.free:
                     break
                 pending.popleft()

@@ -15,7 +15,7 @@ import pytest
 import finitewave_amd as fw
 from finitewave_amd.analysis import apd90
 from finitewave_amd.batch import BatchSim, TissueBatch
-from finitewave_amd.serving import Request, Server
+from finitewave_amd.serving import Request, Server, fibrotic_fraction, fp64_for_fibrosis
 from finitewave_amd.workloads import fibrotic_tissue, restitution_ensemble
 
 
@@ -207,14 +207,15 @@ def test_serving_cancels_queued_and_running_requests(gpu):
         seen.append((r.rid, r.status))
 
     # cancel request 1 once it is running: after the first chunk
-    orig = srv.batches["ap"].advance
+    batch = srv.batch(reqs[0])
+    orig = batch.advance
 
     def advance(n=None):
         out = orig(n)
-        if srv.batches["ap"].step >= 40:
+        if batch.step >= 40:
             srv.cancel(1)
         return out
-    srv.batches["ap"].advance = advance
+    batch.advance = advance
     stats = srv.serve(reqs, realtime=False, log=log)
     srv.close()
     status = {r.rid: r.status for r in reqs}
@@ -478,3 +479,32 @@ def test_multi_step_shapes_on_wide_tissues(gpu, model, prec, kind):
             except AssertionError as e:
                 bad.append(f"variant {v} tissue {k}: {e}")
     assert not bad, "\n".join(bad)
+
+
+@pytest.mark.gpu
+def test_serving_policy_runs_fibrotic_tp06_tissues_on_the_fp64_path(gpu):
+    """fp64_for_fibrosis routes the TP06 tissues at or above the threshold to
+    an FP64 batch and every other request to FP32; the two TP06 batches run
+    side by side, and each request's results equal its plain single-tissue
+    run in the precision it was routed to (for the FP64-routed tissue: the
+    FP64 path the served results are compared with)."""
+    def tp06(rid, p, seed):
+        t = fibrotic_tissue(40, p, seed)
+        sim = BatchSim(tissue=t, t_max=6.0, stims=[fw.StimCurrentCoord2D(0, 100, 1.0, 0, 42, 0, 5)],
+                       probe=(21, int(np.flatnonzero(t.mesh[21] == 1)[0])), act_threshold=-40.0,
+                       apd_threshold=-40.0)
+        return Request(rid=rid, arrival=0.0, model="tp06", sim=sim)
+
+    reqs = [tp06(0, 0.05, 500), tp06(1, 0.2, 501), tp06(2, 0.35, 502), _ap_request(3, 503)]
+    reqs[3].sim.tissue = fibrotic_tissue(30, 0.35, 503)
+    classes = {"tp06": fw.TP062D, "ap": fw.AlievPanfilov2D}
+    srv = Server(classes, precision="fp32", max_tissues=4, chunk_steps=120, hbm_budget=1 << 30,
+                 precision_for=fp64_for_fibrosis(("tp06",), 0.3))
+    stats = srv.serve(reqs, realtime=False)
+    srv.close()
+    assert stats["completed"] == 4
+    assert [r.precision for r in reqs] == ["fp32", "fp32", "fp64", "fp32"]
+    assert fibrotic_fraction(reqs[2].sim) >= 0.3 > fibrotic_fraction(reqs[1].sim)
+    assert set(srv.batches) == {("tp06", "fp32"), ("tp06", "fp64"), ("ap", "fp32")}
+    for r in reqs:
+        _check_same(r.sim.result, _single(classes[r.model], r.precision, r.sim))
