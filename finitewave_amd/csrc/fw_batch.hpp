@@ -188,8 +188,19 @@ __host__ __device__ constexpr int batch_unroll(int variant) {
 }
 
 
+// Waves per SIMD the compiler must leave room for in the single-step batch
+// kernel.  The FP64 TP06 step (exact weights formed per node) would take 132
+// registers (3 waves); held to 4 waves it takes 122 without spills and steps a
+// batch of six fibrotic 400^2 tissues in 185 us against 200 (5 waves would
+// spill 45).  Other instances keep the compiler's choice.
+template <int MODEL, typename R>
+constexpr int batch_min_waves() {
+  return (MODEL == 6 && sizeof(R) == 8) ? 4 : 1;
+}
+
 template <int MODEL, typename R, int V>
-__global__ __launch_bounds__(kBlock) void k_batch_step(const BatchArgs<R, UType<MODEL, R>> a) {
+__global__ __launch_bounds__(kBlock, (batch_min_waves<MODEL, R>())) void k_batch_step(
+    const BatchArgs<R, UType<MODEL, R>> a) {
   using U = UType<MODEL, R>;  // the potential's storage type
   constexpr int KU = batch_unroll(V);
   constexpr bool XCD = (V & kBatchXcd) != 0;
