@@ -255,7 +255,9 @@ class TissueBatch:
         st = sum(8 if f else 4 for f in self.f64)
         if self.alt_state:
             st *= 2
-        return n * (2 * rb + st + 1 + 4) + rb * (horizon_steps + 1)
+        # models 5-7: the library's list of updated nodes (int32, at most one per node)
+        lst = 4 if self.model_id >= 5 else 0
+        return n * (2 * rb + st + 1 + 4 + lst) + rb * (horizon_steps + 1)
 
     # ------------------------------------------------------------------ admission
     @_on_stream

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <vector>
 
 #include "fw_batch.hpp"
 #include "fw_internal.hpp"
@@ -1078,7 +1079,7 @@ static BatchLaunch batch_args(const fw_batch* b, int32_t step, const int32_t* in
   L.step_in = in;
   L.step_out = out;
   L.step = step;
-  L.variant = b->variant & (kBatchVariants - 1);  // the single-step shape
+  L.variant = b->variant & ((kBatchVariants - 1) | kBatchList);  // the single-step shape
   L.D = b->D;
   L.dt = b->dt;
   L.dr2 = b->dr2;
@@ -1216,7 +1217,10 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
   for (int q = 0; q < FW_MAX_PARAMS; ++q) b->P.v[q] = q < n_params ? params[q] : 0.0;
   complete_params(model, b->P);
   b->host = new (std::nothrow) fw_tissue_desc[max_tissues]();
-  if (!b->host || !b->active ||
+  b->lists = new (std::nothrow) int32_t*[max_tissues]();
+  b->list_n = new (std::nothrow) int32_t[max_tissues]();
+  b->list_cap = new (std::nothrow) int64_t[max_tissues]();
+  if (!b->host || !b->active || !b->lists || !b->list_n || !b->list_cap ||
       hipMalloc(&b->table, sizeof(BatchTissue) * max_tissues) != hipSuccess ||
       hipMalloc(&b->step_words, 4 * sizeof(int32_t)) != hipSuccess ||
       hipMalloc(&b->flags, sizeof(int32_t) * max_tissues) != hipSuccess) {
@@ -1247,6 +1251,12 @@ int fw_batch_destroy(fw_batch* b) {
   if (b->bstart) (void)hipFree(b->bstart);
   if (b->step_words) (void)hipFree(b->step_words);
   if (b->flags) (void)hipFree(b->flags);
+  if (b->lists)
+    for (int32_t q = 0; q < b->max_tissues; ++q)
+      if (b->lists[q]) (void)hipFree(b->lists[q]);
+  delete[] b->lists;
+  delete[] b->list_n;
+  delete[] b->list_cap;
   delete[] b->host;
   delete[] b->active;
   delete b;
@@ -1296,8 +1306,45 @@ int fw_batch_set_tissue(fw_batch* b, int32_t slot, const fw_tissue_desc* d, void
               "take no special boundaries", slot);
     return FW_E_INVALID;
   }
+  // models 5-7: the node-list shapes' list of the tissue's updated nodes and
+  // its probe node, ascending, built on the host from the codes (the stream
+  // has drained: no launch still reads the slot's previous list)
+  int32_t n_list = 0;
+  if (b->model >= 5) {
+    std::vector<uint8_t> codes((size_t)d->n);
+    if (hipMemcpyAsync(codes.data(), d->code, (size_t)d->n, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("fw_batch_set_tissue: code copy failed");
+      return FW_E_HIP;
+    }
+    const int64_t pidx = d->probe ? d->probe_idx : -1;
+    std::vector<int32_t> list;
+    list.reserve((size_t)d->n);
+    for (int64_t c = 0; c < d->n; ++c)
+      if ((codes[(size_t)c] & 3) == 1 || c == pidx) list.push_back((int32_t)c);
+    n_list = (int32_t)list.size();
+    if (n_list > b->list_cap[slot]) {
+      if (b->lists[slot]) (void)hipFree(b->lists[slot]);
+      b->lists[slot] = nullptr;
+      b->list_cap[slot] = 0;
+      if (hipMalloc(&b->lists[slot], sizeof(int32_t) * (size_t)n_list) != hipSuccess) {
+        set_error("fw_batch_set_tissue: node list allocation failed");
+        return FW_E_NOMEM;
+      }
+      b->list_cap[slot] = n_list;
+    }
+    if (n_list && (hipMemcpyAsync(b->lists[slot], list.data(), sizeof(int32_t) * (size_t)n_list,
+                                  hipMemcpyHostToDevice, s) != hipSuccess ||
+                   hipStreamSynchronize(s) != hipSuccess)) {
+      set_error("fw_batch_set_tissue: node list copy failed");
+      return FW_E_HIP;
+    }
+  }
   BatchTissue r;
   memset(&r, 0, sizeof(r));
+  r.nodes = b->model >= 5 ? b->lists[slot] : nullptr;
+  r.n_nodes = n_list;
   r.u = d->u;
   r.un = d->un;
   r.code = d->code;
@@ -1326,6 +1373,7 @@ int fw_batch_set_tissue(fw_batch* b, int32_t slot, const fw_tissue_desc* d, void
     return FW_E_HIP;
   }
   b->host[slot] = *d;
+  b->list_n[slot] = n_list;
   return FW_OK;
 }
 
@@ -1358,6 +1406,9 @@ int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity) {
   };
   for (int32_t v = 0; v < kBatchVariants; ++v)
     if (batch_variant_ok(model, v)) add(v);
+  // node-list shapes over each single-step shape (models 5-7)
+  for (int32_t v = 0; v < kBatchVariants; ++v)
+    if (batch_variant_ok(model, v | kBatchList)) add(v | kBatchList);
   // multi-step shapes over the XCD-contiguous 4-node single-step shape, with
   // and without the cache hint
   for (int32_t base : {kBatchVec | kBatchXcd | kBatchNT, kBatchVec | kBatchXcd})
@@ -1386,9 +1437,9 @@ int fw_batch_set_variant(fw_batch* b, int32_t variant, void* stream) {
   const int32_t old = b->variant;
   b->variant = variant;
   if (b->n_active > 0 && (batch_unroll(old) != batch_unroll(variant) ||
-                          ((old | variant) & kBatchTB))) {
-    // the block table's granularity changes with the nodes per work-item,
-    // the unit table with the multi-step shape
+                          ((old | variant) & kBatchTB) || ((old ^ variant) & kBatchList))) {
+    // the block table's granularity changes with the nodes per work-item and
+    // its extent with the node lists, the unit table with the multi-step shape
     int rc = fw_batch_set_active(b, b->active, b->n_active, stream);
     if (rc) {
       b->variant = old;
@@ -1405,6 +1456,9 @@ int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void
     return FW_E_INVALID;
   }
   const int64_t per = (int64_t)batch_unroll(b->variant) * kBlock;
+  const bool listed = (b->variant & kBatchList) != 0;
+  // blocks cover a tissue's nodes, or its node list under the list shapes
+  auto extent = [&](int32_t s) -> int64_t { return listed ? b->list_n[s] : b->host[s].n; };
   int64_t nblocks = 0;
   for (int32_t q = 0; q < n_slots; ++q) {
     const int32_t s = slots[q];
@@ -1412,7 +1466,7 @@ int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void
       set_error("fw_batch_set_active: slot %d is not occupied", s);
       return FW_E_INVALID;
     }
-    nblocks += (b->host[s].n + per - 1) / per;
+    nblocks += (extent(s) + per - 1) / per;
   }
   if (nblocks > INT32_MAX / 2) {
     set_error("fw_batch_set_active: too many blocks");
@@ -1427,7 +1481,7 @@ int fw_batch_set_active(fw_batch* b, const int32_t* slots, int32_t n_slots, void
   int32_t* hs = hb + nblocks;
   int64_t q0 = 0;
   for (int32_t q = 0; q < n_slots; ++q) {
-    const int64_t n = b->host[slots[q]].n;
+    const int64_t n = extent(slots[q]);
     for (int64_t st = 0; st < n; st += per, ++q0) {
       ht[q0] = slots[q];
       hs[q0] = (int32_t)st;

@@ -45,11 +45,12 @@ struct BatchTissue {
   void* probe;
   void* state[FW_MAX_STATE];
   void* state_alt[FW_BATCH_ALT_STATE];  // the multi-step shapes' second state set
+  const int32_t* nodes;  // the node-list shapes: updated nodes and the probe node, ascending
   double dd[4];      // half-step fields (FP64 weights)
   double act_thr;
   float df[4];       // the same fields rounded to float (FP32 flux form)
   float act_thr_f;
-  int32_t n, ni, nj, step0, act_start, probe_idx, probe_len, stencil;
+  int32_t n, ni, nj, step0, act_start, probe_idx, probe_len, stencil, n_nodes;
 };
 
 // A unit of the multi-step launch shape (fw_batch_tb.hpp): tissue, first
@@ -152,6 +153,9 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
 //   bit 2: state and new potential read and written with the non-temporal
 //          hint (faster once a step's data is well beyond the Infinity Cache).
 // A block covers batch_unroll(variant) * kBlock consecutive nodes of a tissue.
+//   bit 8 (models 5-7): the single-step shapes over each tissue's list of
+//          updated nodes (and its probe node) instead of all its nodes, so
+//          no lane of a fibrotic tissue idles through the ionic update;
 //   bit 3 (light models): K = batch_tb_k(variant) steps per launch, one
 //          wavefront per 64 columns x batch_tb_rows(variant) rows of a tissue
 //          (fw_batch_tb.hpp), in pairs of launches; bits 0-2 then give the
@@ -159,6 +163,7 @@ __device__ __forceinline__ R diffuse9(int stencil, const double* dd, const float
 //          and bit 2 also the multi-step launch's cache hint.
 constexpr int kBatchVec = 1, kBatchXcd = 2, kBatchNT = 4, kBatchVariants = 8;
 constexpr int kBatchTB = 8;
+constexpr int kBatchList = 256;
 constexpr int kBatchTbKs = 2, kBatchTbRowSizes = 2;  // K in {2, 4}, rows in {64, 128}
 
 __host__ __device__ constexpr int batch_tb_k(int variant) { return 2 << ((variant >> 4) & 3); }
@@ -172,6 +177,9 @@ __host__ __device__ constexpr bool batch_base_ok(int model, int base) {
 }
 
 __host__ __device__ constexpr bool batch_variant_ok(int model, int variant) {
+  if (variant & kBatchList)
+    return model >= 5 && (variant & ~kBatchList) < kBatchVariants &&
+           batch_base_ok(model, variant & ~kBatchList);
   return (variant & kBatchTB)
              ? (model <= 4 && variant >= 0 && (variant >> 8) == 0 &&
                 ((variant >> 4) & 3) < kBatchTbKs && ((variant >> 6) & 3) < kBatchTbRowSizes &&
@@ -206,8 +214,10 @@ __global__ __launch_bounds__(kBlock, (batch_min_waves<MODEL, R>())) void k_batch
   constexpr bool XCD = (V & kBatchXcd) != 0;
   constexpr bool NT = (V & kBatchNT) != 0;
   constexpr bool EXACT = sizeof(R) == 8;
+  constexpr bool LIST = (V & kBatchList) != 0;
   constexpr int NS = model_n_state(MODEL);
   static_assert(KU == 1 || (KU == 4 && MODEL <= 4), "4 nodes per work-item: light models");
+  static_assert(!LIST || KU == 1, "node lists: one node per work-item");
   const uint32_t b = blockIdx.x;
   const uint32_t lb = XCD ? (b % kXcds) * a.per_xcd + b / kXcds : b;
   if (lb >= a.nblocks) return;
@@ -250,8 +260,15 @@ __global__ __launch_bounds__(kBlock, (batch_min_waves<MODEL, R>())) void k_batch
   const bool aniso = stencil != FW_STENCIL_ISO;
 
   if constexpr (KU == 1) {
-    const int32_t c = start + (int32_t)threadIdx.x;
-    if (c >= n) return;
+    int32_t c = start + (int32_t)threadIdx.x;
+    if constexpr (LIST) {
+      // blocks run over the list; a listed node is updated unless it is
+      // the probe node, which the code check below lets through unchanged
+      if (c >= d.n_nodes) return;
+      c = d.nodes[c];
+    } else {
+      if (c >= n) return;
+    }
     const uint8_t cd = code[c];
     const U uc = u[c];
     if (act_on && act[c] < 0 && uc > thr) act[c] = ls;

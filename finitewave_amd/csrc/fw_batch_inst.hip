@@ -63,6 +63,11 @@ int FW_BLAUNCHER(const BatchLaunch& L, hipStream_t s) {
     FW_BCASE(kBatchVec | kBatchXcd)
     FW_BCASE(kBatchVec | kBatchNT)
     FW_BCASE(kBatchVec | kBatchXcd | kBatchNT)
+#else
+    FW_BCASE(kBatchList)
+    FW_BCASE(kBatchList | kBatchXcd)
+    FW_BCASE(kBatchList | kBatchNT)
+    FW_BCASE(kBatchList | kBatchXcd | kBatchNT)
 #endif
 #undef FW_BCASE
     default:

@@ -125,7 +125,7 @@ struct BatchLaunch {
   const int32_t* step_in;
   int32_t* step_out;
   int32_t step;
-  int32_t variant;  // launch shape (fw_batch.hpp: kBatchVec | kBatchXcd)
+  int32_t variant;  // launch shape (fw_batch.hpp: kBatchVec | kBatchXcd | kBatchNT | kBatchList)
   double D, dt, dr2, inv_dr2;
   Params P;
 };
@@ -164,4 +164,9 @@ struct fw_batch {
   hipGraphExec_t tb_graph;       // one pair of multi-step launches
   hipGraphExec_t tb_graph_long;  // kBatchTbLongPairs pairs
   int32_t tb_graph_units, tb_graph_iso, tb_graph_variant;
+  // node-list shapes (models 5-7): per slot, the device list of its updated
+  // nodes and probe node, its length and the buffer's capacity (entries)
+  int32_t** lists;
+  int32_t* list_n;
+  int64_t* list_cap;
 };

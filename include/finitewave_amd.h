@@ -391,7 +391,10 @@ int fw_batch_create(int32_t model, int32_t precision, double D_model, double dt,
 /* Frees the batch; no launch of it may still be pending (synchronize the
  * stream its calls used first). */
 int fw_batch_destroy(fw_batch* batch);
-/* Writes slot's descriptor (host struct copied to the device table). */
+/* Writes slot's descriptor (host struct copied to the device table).  For
+ * models 5-7 it also builds the slot's node list (its updated nodes and its
+ * probe node, int32, ascending) from the codes, in device memory the batch
+ * owns and reuses; the slot must not be in use by a pending launch. */
 int fw_batch_set_tissue(fw_batch* batch, int32_t slot, const fw_tissue_desc* desc,
                         void* stream);
 /* Sets the tissues that fw_batch_run steps: `slots` (host, n_slots entries)
@@ -411,12 +414,14 @@ int fw_batch_block_nodes(const fw_batch* batch);
  * accessed with the non-temporal cache hint.  Bit 3 (models 0-4) = K steps
  * per launch, K = 2 << bits 4-5, each wavefront marching down 64 columns by
  * 64 << bits 6-7 rows of a tissue; steps are run as pairs of such launches
- * (2K steps), the rest with the shape of bits 0-2.  Writes up to `capacity`
- * of them to `variants` and returns how many there are. */
+ * (2K steps), the rest with the shape of bits 0-2.  Bit 8 (models 5-7) =
+ * the shape of bits 1-2 over each tissue's node list instead of all its nodes
+ * (no lane idles on a non-updated node).  Writes up to `capacity` of them to
+ * `variants` and returns how many there are. */
 int fw_batch_variants(int32_t model, int32_t* variants, int32_t capacity);
 /* Selects the launch shape of fw_batch_run (rebuilds the block table when the
- * nodes per work-item change); returns FW_E_INVALID for a variant the model
- * does not have. */
+ * nodes per work-item or the node lists change); returns FW_E_INVALID for a
+ * variant the model does not have. */
 int fw_batch_set_variant(fw_batch* batch, int32_t variant, void* stream);
 int fw_batch_get_variant(const fw_batch* batch);
 /* n_steps global steps from step0; use_graph replays captured graphs

@@ -405,8 +405,10 @@ def test_batch_launch_variants_per_model():
     # for K = 2, 4 (bits 4-5) and runs of 64, 128 rows (bits 6-7) over the
     # 4-node XCD-contiguous shape with and without the cache hint
     multi = [base | 8 | (k << 4) | (r << 6) for base in (7, 3) for k in (0, 1) for r in (0, 1)]
+    # models 5-7: the four one-node shapes, then the same over node lists (bit 8)
+    heavy = [0, 2, 4, 6, 256, 258, 260, 262]
     for model, expect in ((2, list(range(8)) + multi), (0, list(range(8)) + multi),
-                          (6, [0, 2, 4, 6]), (7, [0, 2, 4, 6])):
+                          (5, heavy), (6, heavy), (7, heavy)):
         arr = (ctypes.c_int32 * 32)()
         n = L.fw_batch_variants(model, arr, 32)
         assert list(arr[:n]) == expect
