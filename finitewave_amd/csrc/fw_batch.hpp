@@ -187,8 +187,11 @@ __host__ __device__ constexpr bool batch_variant_ok(int model, int variant) {
              : batch_base_ok(model, variant);
 }
 
+// the shape without autotuning: for models 5-7 the XCD map without the cache
+// hint, which TP06 batches ran fastest with in both precisions (six 400^2
+// tissues, FP64: 136.6 against 185 us per step with the hint)
 __host__ __device__ constexpr int batch_default_variant(int model) {
-  return model <= 4 ? (kBatchVec | kBatchXcd | kBatchNT) : (kBatchXcd | kBatchNT);
+  return model <= 4 ? (kBatchVec | kBatchXcd | kBatchNT) : kBatchXcd;
 }
 
 __host__ __device__ constexpr int batch_unroll(int variant) {
