@@ -274,7 +274,7 @@ __global__ __launch_bounds__(kBlock, (batch_min_waves<MODEL, R>())) void k_batch
     }
     const uint8_t cd = code[c];
     const U uc = u[c];
-    if (act_on && act[c] < 0 && uc > thr) act[c] = ls;
+    if (act_on && uc > thr && act[c] < 0) act[c] = ls;
     if (probe_on && c == probe_idx) probe[ls] = uc;
     // rows 0 and ni-1 would reach outside the grid
     if ((cd & 3) != 1 || c < nj || c >= n - nj) return;
@@ -372,7 +372,7 @@ __global__ __launch_bounds__(kBlock, (batch_min_waves<MODEL, R>())) void k_batch
       if (cc >= n) break;
       const uint8_t cd = code[cc];
       const R uc = u[cc];
-      if (act_on && act[cc] < 0 && uc > thr) act[cc] = ls;
+      if (act_on && uc > thr && act[cc] < 0) act[cc] = ls;
       if (probe_on && cc == probe_idx) probe[ls] = uc;
       if ((cd & 3) != 1 || cc < nj || cc >= n - nj) continue;
       Nb9<R> q;

@@ -996,7 +996,7 @@ __device__ __forceinline__ void step_node(const KArgs<R, UType<MODEL, R>>& a, in
   const uint8_t code = a.code[c];
   const U uc = a.u[c];
   if (a.act_on) {
-    if (a.act[c] < 0 && uc > a.act_thr) a.act[c] = step;
+    if (uc > a.act_thr && a.act[c] < 0) a.act[c] = step;
   }
   if ((code & 3) == 1) {
     // node coordinates; skipped work for nodes on the outer layer keeps every

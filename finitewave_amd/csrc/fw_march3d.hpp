@@ -171,8 +171,11 @@ __global__ __launch_bounds__(kBlock) void k_step3_march(const KArgs<float, float
     const float fz0 = from_west(fz);
     if (own && i >= i0) {
       const float uc = u1;
+      // the activation word is read only for a potential above the
+      // threshold: below it nothing can be recorded, and a read whose
+      // result the loop waits on costs every plane its latency
       if (a.act_on) {
-        if (a.act[c] < 0 && uc > a.act_thr) a.act[c] = step;
+        if (uc > a.act_thr && a.act[c] < 0) a.act[c] = step;
       }
       if ((cd1 & 3) == 1 && row_interior && i >= 1 && i <= a.ni - 2) {
         const float fl = (fx - fx_lo) + (fy - fy0) + (fz - fz0);
